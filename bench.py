@@ -1,0 +1,135 @@
+"""Throughput of the FP64 ETO-HEOM RK4 propagation on one MI355X.
+
+A bench "step" is one whole propagation of the workload (every RK4 step of it,
+V(t) built and P(t) read out on the device), with the hierarchy state resident
+in HBM.  The default workload is the heptamer: ground + 7 singles + 21 doubles
+(N=29), 2 ETO modes/site (d=14), depth L=4 (3060 ADOs), 2000 RK4 steps of
+0.5 fs, 3 Gaussian pulses; `value` is ADO-updates/s (one ADO advanced one full
+RK4 step).  Other workloads: --workload dimer | grid2d | chain10 | trimer.
+
+Printed as one JSON line with the roofline of the dominant kernel (the fused
+RHS+RK4 stage kernel, timed per launch with HIP events on its own stream) and
+the CPU oracle timed on this host over a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "eto-heom_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def workload(name):
+    from etoheom_amd import configs
+    if name == "heptamer":
+        return [configs.heptamer()], "heptamer N=29 (ground+7+21), d=14, L=4, 3060 ADOs, 2000 RK4 steps dt=0.5 fs"
+    if name == "dimer":
+        return [configs.dimer()], "dimer N=4, d=4, L=6, 210 ADOs, 2000 RK4 steps dt=0.5 fs"
+    if name == "trimer":
+        return [configs.aggregate(3, 5)], "trimer N=7, d=6, L=5, 462 ADOs, 2000 RK4 steps"
+    if name == "chain10":
+        return [configs.chain()], "10-site chain N=56, d=20, L=3, 1771 ADOs, 3000 RK4 steps"
+    if name == "grid2d":
+        return configs.grid2d(), "512 propagations N=4/7/11, L=3..6, static disorder, 2000 RK4 steps each"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(p, seconds=20.0):
+    """The oracle (a scalar FP64 port of the reference CPU path) on one core,
+    over whole RK4 steps of the same instance until ~`seconds` have elapsed."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    q = O.Problem(H=p.H, X=p.X, alpha=p.alpha, alpha_t=p.alpha_t, gamma=p.gamma, L=p.L,
+                  site_offset=p.site_offset, pulses=p.pulses, t_start=p.t_start, t_end=p.t_end,
+                  step_size=p.step_size, print_step=p.print_step)
+    M = q.num_ado()
+    steps, elapsed = 1, 0.0
+    while True:
+        t0 = time.perf_counter()
+        O.twod(q, nv=(1, 1, 1), max_steps=steps)
+        elapsed = time.perf_counter() - t0
+        if elapsed > seconds / 3 or steps >= 4096:
+            break
+        steps *= 4 if elapsed < seconds / 30 else 2
+    return {"value": M * steps / elapsed, "unit": "ADO-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} RK4 steps of the same instance ({M} ADOs), {elapsed:.1f} s, 1 thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="heptamer")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=20)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}); independent propagations shard "
+              "across GPUs with no communication but the final gather of P(t).", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+    import etoheom_amd as E
+
+    items, desc = workload(a.workload)
+    ctx = E.Context(0)
+    batch = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)} for p in items])
+    for _ in range(a.warmup):
+        batch.run()
+        batch.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        batch.run()
+    batch.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = batch.elapsed_ms()
+    outs = [batch.output(i) for i in range(len(items))]
+    finite = all(np.all(np.isfinite(o)) for o in outs)
+
+    # per-launch durations of the stage kernel (HIP events on the batch stream)
+    prof = batch.profile(a.profile_steps)
+    stage_ms = float(np.mean(prof[1:5]))
+    elems = sum(p.num_ado() * p.N * p.N for p in items)
+    bytes_per_launch = 64.0 * elems        # 256 N^2 B per ADO-update over 4 stage launches
+    achieved = bytes_per_launch / (stage_ms * 1e-3) / 1e9
+
+    ups = batch.ado_updates * a.steps
+    value = ups / wall
+    rec = {
+        "metric": "FP64 ADO-updates/s + 2DES grid points/s at 1e-10 err; adjoint-grad time vs FD",
+        "value": value,
+        "unit": "ADO-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": desc, "propagations_per_step": len(items),
+                   "propagations_per_s": len(items) * a.steps / wall,
+                   "grid_points_per_s": len(items) * a.steps / wall / 3.0,
+                   "gpu_event_ms_last_run": gpu_ms, "device": ctx.name, "finite": finite},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "stage (fused RHS + RK4 stage)", "stage_ms": [float(x) for x in prof[1:5]],
+                     "prologue_ms": float(prof[0]), "bytes_per_launch": bytes_per_launch},
+    }
+    if not a.no_cpu:
+        rec["cpu_baseline"] = cpu_baseline(items[0])
+    print(json.dumps(rec))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,786 @@
+// engine.hip -- the batched FP64 HEOM RK4 propagator on CDNA4 (gfx950).
+//
+// Replaces the hot loops of twoD_spectrum_solver / dynamics_solver
+// (etoheom-cpu/lib/dynamics.cpp:202-432) and their RHS total_ADO_dynamics[_Ht]
+// (:53-200).  Design:
+//
+//  * Every instance (one propagation) owns a contiguous slice [M][N][N] of four
+//    complex128 state buffers: Y (the state), ACC (the RK4 accumulator) and two
+//    stage-input buffers S1/S2.  A step is a prologue kernel (V(t_i) and the
+//    readout of Tr(rho_0 R) before the step) and four fused RHS+RK4 stage kernels:
+//      stage 0: k1 = f(Y)   ACC = Y + h/6 k1     S1 = Y + h/2 k1
+//      stage 1: k2 = f(S1)  ACC += h/3 k2        S2 = Y + h/2 k2
+//      stage 2: k3 = f(S2)  ACC += h/3 k3        S1 = Y + h k3
+//      stage 3: k4 = f(S1)  Y = ACC + h/6 k4
+//    which is the reference's in-place RK4 (dynamics.cpp:385-417) with V frozen
+//    over the step, and 9 reads + 7 writes of each ADO per step.
+//  * f(rho)_n[a][b] = -i (V rho_n - rho_n V)[a][b] + damp_n rho_n[a][b]
+//                     + sum_links ([a==s] w_row + [b==s] w_col) rho_m[a][b]
+//    i.e. the one-hot S_j products of the reference reduce to element gathers of
+//    the same (a, b) entry of the neighbour ADO.
+//  * Small N (<= 16): one workgroup per tile of G = 512/N^2 ADOs, the tile and V
+//    staged in LDS, one thread per matrix element.  Larger N: one ADO per
+//    workgroup with the commutator from LDS (the MFMA path replaces it).
+//  * The time loop is captured as a HIP graph of `chunk` steps; kernels read the
+//    step index from a device counter that a one-thread kernel advances, so one
+//    instantiated graph is replayed for the whole propagation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace eh {
+
+// ---------------------------------------------------------------- device data
+
+struct InstDev {
+    long long state_off;   // complex offset of ADO 0 in the state buffers
+    long long p_off;       // complex offset in the readout buffer
+    int N, M;
+    int ado_off;           // offset in the global ADO numbering (CSR, damping)
+    int nsteps;
+    int v_off;             // complex offset of this instance's N*N blocks (H, V, X'_q, R_r)
+    int step_off;          // offset in the per-step tables
+    int nread;             // readout operators
+    int npoints;
+    int real_h;            // H has no imaginary part
+    int r_off;             // complex offset of this instance's readout operators
+    double h2, h, hb[4];
+};
+
+struct StageArgs {
+    const InstDev* inst;
+    const int* tile_inst;
+    const int* tile_ado0;
+    const int* tile_nado;
+    const int* step_ctr;
+    int step_off;
+    int stage;
+    const uint8_t* vreal;    // per step: V(t_i) has no imaginary part
+    const double2* V;        // per instance N*N
+    const int* link_ptr;     // per global ADO + 1
+    const int* link_nbr;
+    const int* link_site;
+    const double2* link_wr;
+    const double2* link_wc;
+    const double2* damp;     // per global ADO
+    const double2* rin;      // stage input
+    const double2* y;        // Y
+    double2* acc;            // ACC
+    double2* out;            // stage output
+};
+
+__device__ __forceinline__ double2 cmad(double2 acc, double2 a, double2 b) {
+    acc.x = fma(a.x, b.x, fma(-a.y, b.y, acc.x));
+    acc.y = fma(a.x, b.y, fma(a.y, b.x, acc.y));
+    return acc;
+}
+
+// RHS value k at global element gi -> RK4 update for this stage
+__device__ __forceinline__ void rk_update(const StageArgs& A, const InstDev& I, size_t gi, double2 rin_val,
+                                          double kr, double ki) {
+    switch (A.stage) {
+        case 0: {
+            const double2 y = rin_val;
+            A.acc[gi] = make_double2(fma(I.hb[0], kr, y.x), fma(I.hb[0], ki, y.y));
+            A.out[gi] = make_double2(fma(I.h2, kr, y.x), fma(I.h2, ki, y.y));
+            break;
+        }
+        case 1:
+        case 2: {
+            const double2 y = A.y[gi];
+            double2 a = A.acc[gi];
+            const double hb = I.hb[A.stage];
+            const double hs = A.stage == 1 ? I.h2 : I.h;
+            a.x = fma(hb, kr, a.x);
+            a.y = fma(hb, ki, a.y);
+            A.acc[gi] = a;
+            A.out[gi] = make_double2(fma(hs, kr, y.x), fma(hs, ki, y.y));
+            break;
+        }
+        default: {
+            const double2 a = A.acc[gi];
+            A.out[gi] = make_double2(fma(I.hb[3], kr, a.x), fma(I.hb[3], ki, a.y));
+            break;
+        }
+    }
+}
+
+// coupling gathers for element (ra, cb) of ADO gado; adds into (dr, di)
+__device__ __forceinline__ void add_links(const StageArgs& A, const InstDev& I, int gado, int ra, int cb, int ab,
+                                          int NN, double& dr, double& di) {
+    const int l0 = A.link_ptr[gado], l1 = A.link_ptr[gado + 1];
+    for (int l = l0; l < l1; ++l) {
+        const int s = A.link_site[l];
+        const bool rw = (ra == s), cl = (cb == s);
+        if (rw | cl) {
+            double2 w = make_double2(0.0, 0.0);
+            if (rw) w = A.link_wr[l];
+            if (cl) {
+                const double2 c = A.link_wc[l];
+                w.x += c.x;
+                w.y += c.y;
+            }
+            const double2 x = A.rin[I.state_off + (long long)A.link_nbr[l] * NN + ab];
+            dr = fma(w.x, x.x, fma(-w.y, x.y, dr));
+            di = fma(w.x, x.y, fma(w.y, x.x, di));
+        }
+    }
+}
+
+template <int NT>
+__global__ __launch_bounds__(256) void stage_small(StageArgs A) {
+    constexpr int NN = NT * NT;
+    constexpr int G = NN >= 512 ? 1 : 512 / NN;
+    __shared__ double2 sR[G * NN];
+    __shared__ double2 sV[NN];
+    const int tile = blockIdx.x;
+    const InstDev I = A.inst[A.tile_inst[tile]];
+    const int step = *A.step_ctr + A.step_off;
+    if (step >= I.nsteps) return;
+    const int a0 = A.tile_ado0[tile], na = A.tile_nado[tile];
+    const double2* V = A.V + I.v_off;
+    for (int e = threadIdx.x; e < NN; e += blockDim.x) sV[e] = V[e];
+    const long long base = I.state_off + (long long)a0 * NN;
+    const int nel = na * NN;
+    for (int e = threadIdx.x; e < nel; e += blockDim.x) sR[e] = A.rin[base + e];
+    __syncthreads();
+    const bool vreal = A.vreal[I.step_off + step] != 0;
+    for (int e = threadIdx.x; e < nel; e += blockDim.x) {
+        const int nl = e / NN, ab = e - nl * NN, ra = ab / NT, cb = ab - ra * NT;
+        const double2* R = sR + nl * NN;
+        double cr = 0.0, ci = 0.0;   // (V rho - rho V)[ra][cb]
+        if (vreal) {
+#pragma unroll
+            for (int c = 0; c < NT; ++c) {
+                const double vac = sV[ra * NT + c].x, vcb = sV[c * NT + cb].x;
+                const double2 rcb = R[c * NT + cb], rac = R[ra * NT + c];
+                cr = fma(vac, rcb.x, fma(-rac.x, vcb, cr));
+                ci = fma(vac, rcb.y, fma(-rac.y, vcb, ci));
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < NT; ++c) {
+                const double2 vac = sV[ra * NT + c], vcb = sV[c * NT + cb];
+                const double2 rcb = R[c * NT + cb], rac = R[ra * NT + c];
+                cr = fma(vac.x, rcb.x, fma(-vac.y, rcb.y, cr));
+                ci = fma(vac.x, rcb.y, fma(vac.y, rcb.x, ci));
+                cr = fma(-rac.x, vcb.x, fma(rac.y, vcb.y, cr));
+                ci = fma(-rac.x, vcb.y, fma(-rac.y, vcb.x, ci));
+            }
+        }
+        double dr = ci, di = -cr;   // -i * comm
+        const int gado = I.ado_off + a0 + nl;
+        const double2 g = A.damp[gado], r = R[ab];
+        dr = fma(g.x, r.x, fma(-g.y, r.y, dr));
+        di = fma(g.x, r.y, fma(g.y, r.x, di));
+        add_links(A, I, gado, ra, cb, ab, NN, dr, di);
+        rk_update(A, I, (size_t)(base + e), r, dr, di);
+    }
+}
+
+// any N: one ADO per workgroup, rho_n staged in dynamic LDS, V read through the cache
+__global__ __launch_bounds__(256) void stage_generic(StageArgs A) {
+    extern __shared__ double2 sR[];
+    const int tile = blockIdx.x;
+    const InstDev I = A.inst[A.tile_inst[tile]];
+    const int step = *A.step_ctr + A.step_off;
+    if (step >= I.nsteps) return;
+    const int N = I.N, NN = N * N;
+    const int a0 = A.tile_ado0[tile];
+    const double2* V = A.V + I.v_off;
+    const long long base = I.state_off + (long long)a0 * NN;
+    for (int e = threadIdx.x; e < NN; e += blockDim.x) sR[e] = A.rin[base + e];
+    __syncthreads();
+    const bool vreal = A.vreal[I.step_off + step] != 0;
+    const int gado = I.ado_off + a0;
+    const double2 g = A.damp[gado];
+    for (int ab = threadIdx.x; ab < NN; ab += blockDim.x) {
+        const int ra = ab / N, cb = ab - ra * N;
+        double cr = 0.0, ci = 0.0;
+        if (vreal) {
+            for (int c = 0; c < N; ++c) {
+                const double vac = V[ra * N + c].x, vcb = V[c * N + cb].x;
+                const double2 rcb = sR[c * N + cb], rac = sR[ra * N + c];
+                cr = fma(vac, rcb.x, fma(-rac.x, vcb, cr));
+                ci = fma(vac, rcb.y, fma(-rac.y, vcb, ci));
+            }
+        } else {
+            for (int c = 0; c < N; ++c) {
+                const double2 vac = V[ra * N + c], vcb = V[c * N + cb];
+                const double2 rcb = sR[c * N + cb], rac = sR[ra * N + c];
+                cr = fma(vac.x, rcb.x, fma(-vac.y, rcb.y, cr));
+                ci = fma(vac.x, rcb.y, fma(vac.y, rcb.x, ci));
+                cr = fma(-rac.x, vcb.x, fma(rac.y, vcb.y, cr));
+                ci = fma(-rac.x, vcb.y, fma(-rac.y, vcb.x, ci));
+            }
+        }
+        double dr = ci, di = -cr;
+        const double2 r = sR[ab];
+        dr = fma(g.x, r.x, fma(-g.y, r.y, dr));
+        di = fma(g.x, r.y, fma(g.y, r.x, di));
+        add_links(A, I, gado, ra, cb, ab, NN, dr, di);
+        rk_update(A, I, (size_t)(base + ab), r, dr, di);
+    }
+}
+
+struct PrologueArgs {
+    const InstDev* inst;
+    const int* step_ctr;
+    int step_off;
+    const double2* H;        // per instance N*N
+    const double2* Xp;       // per instance 3*N*N: X[0], X[1]^T, X[2]^T
+    const double2* field;    // per step 3 scalars
+    const int* pidx;         // per step readout slot
+    const double2* R;        // per instance nread*N*N readout operators
+    const double2* y;
+    double2* V;
+    double2* P;
+};
+
+// V(t_i) = H - e_1 X[0] - e_2 X[1]^T - e_3 X[2]^T  (dynamics.cpp:348-363), and
+// the readouts Tr(rho_0 R_r) before step i (dynamics.cpp:367-381, :246-253)
+__global__ __launch_bounds__(256) void prologue(PrologueArgs A) {
+    const InstDev I = A.inst[blockIdx.x];
+    const int step = *A.step_ctr + A.step_off;
+    if (step >= I.nsteps) return;
+    const int N = I.N, NN = N * N;
+    const double2* e = A.field + 3 * (size_t)(I.step_off + step);
+    const double2 e0 = e[0], e1 = e[1], e2 = e[2];
+    const double2* H = A.H + I.v_off;
+    const double2* X = A.Xp + 3 * (size_t)I.v_off;
+    double2* V = A.V + I.v_off;
+    for (int q = threadIdx.x; q < NN; q += blockDim.x) {
+        double2 v = H[q];
+        const double2 x0 = X[q], x1 = X[NN + q], x2 = X[2 * NN + q];
+        v.x -= x0.x * e0.x - x0.y * e0.y;
+        v.y -= x0.x * e0.y + x0.y * e0.x;
+        v.x -= x1.x * e1.x - x1.y * e1.y;
+        v.y -= x1.x * e1.y + x1.y * e1.x;
+        v.x -= x2.x * e2.x - x2.y * e2.y;
+        v.y -= x2.x * e2.y + x2.y * e2.x;
+        V[q] = v;
+    }
+    const int slot = A.pidx[I.step_off + step];
+    if (slot < 0) return;
+    __shared__ double2 red[256];
+    const double2* rho0 = A.y + I.state_off;
+    for (int r = 0; r < I.nread; ++r) {
+        const double2* R = A.R + I.r_off + (size_t)r * NN;
+        double2 s = make_double2(0.0, 0.0);
+        for (int q = threadIdx.x; q < NN; q += blockDim.x) {
+            const int j = q / N, k = q - j * N;
+            s = cmad(s, rho0[q], R[k * N + j]);
+        }
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) {
+                red[threadIdx.x].x += red[threadIdx.x + w].x;
+                red[threadIdx.x].y += red[threadIdx.x + w].y;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) A.P[I.p_off + (long long)r * I.npoints + slot] = red[0];
+        __syncthreads();
+    }
+}
+
+__global__ void tick(int* ctr, int by) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *ctr += by;
+}
+
+// rho_0 = |0><0| for every instance, all other ADOs zero; step counter to 0
+__global__ void reset_state(double2* y, long long total, const long long* ado0_off, int n_inst, int* ctr) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long e = i; e < total; e += stride) y[e] = make_double2(0.0, 0.0);
+    if (i == 0) *ctr = 0;
+}
+__global__ void set_ground(double2* y, const long long* ado0_off, int n_inst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_inst) y[ado0_off[i]] = make_double2(1.0, 0.0);
+}
+
+}  // namespace eh
+
+// ================================================================ host side
+
+namespace eh {
+
+namespace {
+
+#define HIPCHK(expr)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(EH_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t count) {
+        n = count;
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(e == hipErrorOutOfMemory ? EH_ENOMEM : EH_EHIP,
+                        std::string("hipMalloc: ") + hipGetErrorString(e));
+        }
+        return EH_OK;
+    }
+    int upload(const std::vector<T>& h) {
+        int rc = alloc(h.size());
+        if (rc) return rc;
+        if (!h.empty()) HIPCHK(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+        return EH_OK;
+    }
+};
+
+struct Group {
+    int nt;           // compile-time N of stage_small, 0 = stage_generic
+    int tile_begin, tile_count;
+    size_t shmem;
+};
+
+}  // namespace
+
+struct BatchImpl {
+    hipStream_t stream = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevBuf<InstDev> inst;
+    DevBuf<int> tile_inst, tile_ado0, tile_nado, step_ctr, link_ptr, link_nbr, link_site, pidx;
+    DevBuf<uint8_t> vreal;
+    DevBuf<double2> V, H, Xp, field, R, P, wr, wc, damp, Y, ACC, S1, S2;
+    DevBuf<long long> ado0_off;
+    std::vector<Group> groups;
+    int nchunks = 0;
+    bool ran = false;
+
+    ~BatchImpl() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+int ctx_init(int device, eh_ctx* ctx) {
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) return fail(EH_EHIP, "no HIP device available");
+    if (device < 0 || device >= count) return fail(EH_EINVAL, "device index out of range");
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    ctx->device = device;
+    std::snprintf(ctx->name, sizeof(ctx->name), "%s (%s)", prop.name, prop.gcnArchName);
+    ctx->n_cu = prop.multiProcessorCount;
+    ctx->lds_per_block = (int)prop.sharedMemPerBlock;
+    return EH_OK;
+}
+
+static inline double2 d2(cplx c) { return make_double2(c.real(), c.imag()); }
+
+static int launch_stage(const Group& g, const StageArgs& base, int tile0, hipStream_t s) {
+    StageArgs a = base;
+    a.tile_inst += tile0;
+    a.tile_ado0 += tile0;
+    a.tile_nado += tile0;
+    const dim3 grid(g.tile_count), block(256);
+    switch (g.nt) {
+#define EH_CASE(n) \
+    case n: hipLaunchKernelGGL(stage_small<n>, grid, block, 0, s, a); break;
+        EH_CASE(1) EH_CASE(2) EH_CASE(3) EH_CASE(4) EH_CASE(5) EH_CASE(6) EH_CASE(7) EH_CASE(8)
+        EH_CASE(9) EH_CASE(10) EH_CASE(11) EH_CASE(12) EH_CASE(13) EH_CASE(14) EH_CASE(15) EH_CASE(16)
+#undef EH_CASE
+        default: hipLaunchKernelGGL(stage_generic, grid, block, g.shmem, s, a); break;
+    }
+    HIPCHK(hipGetLastError());
+    return EH_OK;
+}
+
+int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
+    if (!ctx || !in || n <= 0 || !out) return fail(EH_EINVAL, "batch_create: bad arguments");
+    HIPCHK(hipSetDevice(ctx->device));
+    std::unique_ptr<Batch> b(new Batch());
+    std::unique_ptr<BatchImpl> im(new BatchImpl());
+    b->ctx = ctx;
+    b->n_inst = n;
+    b->info.resize(n);
+
+    std::vector<InstDev> hinst(n);
+    std::vector<int> t_inst, t_ado0, t_nado;
+    std::vector<int> lptr, lnbr, lsite, hpidx;
+    std::vector<uint8_t> hvreal;
+    std::vector<double2> hH, hXp, hfield, hR, hwr, hwc, hdamp;
+    std::vector<long long> hado0;
+    lptr.push_back(0);
+    int64_t state_off = 0, p_off = 0;
+    int ado_off = 0, v_off = 0, r_off = 0, step_off = 0;
+    // tiles grouped by kernel variant
+    std::vector<std::vector<int>> tiles_by_nt(17);   // each entry: triples (inst, ado0, nado)
+
+    for (int i = 0; i < n; i++) {
+        const eh_instance& q = in[i];
+        if (!q.sys || !q.H) return fail(EH_EINVAL, "instance without system or H");
+        const eh_system& sys = *q.sys;
+        const int N = sys.N;
+        if (N <= 0 || N > 64) return fail(EH_ESHAPE, "N must be in 1..64");
+        if (q.mode == EH_MODE_2DES && !q.X4) return fail(EH_EINVAL, "2DES instance needs X4");
+        if (q.mode == EH_MODE_DYNAMICS && (q.n_sites < 0 || (q.n_sites > 0 && !q.sites)))
+            return fail(EH_EINVAL, "dynamics instance: bad sites");
+        if (sys.K < 0 || sys.Km < 0 || sys.K + sys.site_offset > N) return fail(EH_ESHAPE, "bath sites exceed N");
+        Hierarchy h;
+        int rc = build_hierarchy(sys.K * sys.Km, sys.L, h);
+        if (rc) return rc;
+        Couplings c;
+        rc = build_couplings(h, sys, c);
+        if (rc) return rc;
+        Schedule s;
+        rc = build_schedule(sys, q.mode, q.nv, s);
+        if (rc) return rc;
+        const int NN = N * N;
+        const int M = (int)h.M;
+        // readout operators
+        int nread = 0;
+        if (q.mode == EH_MODE_2DES) {
+            nread = 1;
+            for (int e = 0; e < NN; e++) hR.push_back(make_double2(q.X4[2 * (3 * NN + e)], q.X4[2 * (3 * NN + e) + 1]));
+        } else {
+            if (q.n_sites > N) return fail(EH_ERANGE, "number of output sites exceeds the system size");
+            nread = q.n_sites;
+            for (int r = 0; r < q.n_sites; r++) {
+                const int st = q.sites[r];
+                if (st < 1 || st > N) return fail(EH_ERANGE, "site " + std::to_string(st) + " is out of range");
+                for (int e = 0; e < NN; e++) hR.push_back(make_double2(e == (st - 1) * N + (st - 1) ? 1.0 : 0.0, 0.0));
+            }
+        }
+        bool real_h = true;
+        for (int e = 0; e < NN; e++) {
+            hH.push_back(make_double2(q.H[2 * e], q.H[2 * e + 1]));
+            if (q.H[2 * e + 1] != 0.0) real_h = false;
+        }
+        for (int qq = 0; qq < 3; qq++)
+            for (int a = 0; a < N; a++)
+                for (int bb = 0; bb < N; bb++) {
+                    if (q.mode != EH_MODE_2DES) {
+                        hXp.push_back(make_double2(0.0, 0.0));
+                        continue;
+                    }
+                    const int e = qq == 0 ? a * N + bb : bb * N + a;   // X[1], X[2] transposed
+                    hXp.push_back(make_double2(q.X4[2 * (qq * NN + e)], q.X4[2 * (qq * NN + e) + 1]));
+                }
+        for (int st = 0; st < s.nsteps; st++) {
+            for (int qq = 0; qq < 3; qq++) hfield.push_back(d2(s.e[(size_t)3 * st + qq]));
+            hpidx.push_back(s.pidx[(size_t)st]);
+            hvreal.push_back((real_h && !s.active[(size_t)st]) ? 1 : 0);
+        }
+        for (int m = 0; m < M; m++) {
+            hdamp.push_back(d2(c.damp[(size_t)m]));
+            for (int l = c.ptr[(size_t)m]; l < c.ptr[(size_t)m + 1]; l++) {
+                const Link& lk = c.links[(size_t)l];
+                lnbr.push_back(lk.nbr);
+                lsite.push_back(lk.site);
+                hwr.push_back(d2(lk.w_row));
+                hwc.push_back(d2(lk.w_col));
+            }
+            lptr.push_back((int)lnbr.size());
+        }
+        InstDev& d = hinst[i];
+        d.state_off = state_off;
+        d.p_off = p_off;
+        d.N = N;
+        d.M = M;
+        d.ado_off = ado_off;
+        d.nsteps = s.nsteps;
+        d.v_off = v_off;
+        d.step_off = step_off;
+        d.nread = nread;
+        d.npoints = s.npoints;
+        d.real_h = real_h ? 1 : 0;
+        d.r_off = r_off;
+        d.h2 = s.h2;
+        d.h = s.h;
+        for (int k = 0; k < 4; k++) d.hb[k] = s.hb[k];
+        hado0.push_back(state_off);
+        Batch::Info& inf = b->info[(size_t)i];
+        inf.N = N;
+        inf.M = M;
+        inf.nsteps = s.nsteps;
+        inf.npoints = s.npoints;
+        inf.nread = nread;
+        inf.mode = q.mode;
+        inf.state_off = state_off;
+        inf.p_off = p_off;
+        b->ado_updates += (int64_t)M * s.nsteps;
+        b->max_steps = std::max(b->max_steps, s.nsteps);
+        // tiles
+        const int nt = N <= 16 ? N : 0;
+        const int G = nt ? (NN >= 512 ? 1 : 512 / NN) : 1;
+        for (int a0 = 0; a0 < M; a0 += G) {
+            auto& tv = tiles_by_nt[(size_t)nt];
+            tv.push_back(i);
+            tv.push_back(a0);
+            tv.push_back(std::min(G, M - a0));
+        }
+        state_off += (int64_t)M * NN;
+        p_off += (int64_t)nread * s.npoints;
+        ado_off += M;
+        v_off += NN;
+        r_off += nread * NN;
+        step_off += s.nsteps;
+    }
+    b->total_ados = ado_off;
+    b->total_elems = state_off;
+    for (int nt = 0; nt <= 16; nt++) {
+        const auto& tv = tiles_by_nt[(size_t)nt];
+        if (tv.empty()) continue;
+        Group g;
+        g.nt = nt;
+        g.tile_begin = (int)t_inst.size();
+        g.tile_count = (int)(tv.size() / 3);
+        int maxnn = 0;
+        for (size_t k = 0; k < tv.size(); k += 3) {
+            t_inst.push_back(tv[k]);
+            t_ado0.push_back(tv[k + 1]);
+            t_nado.push_back(tv[k + 2]);
+            maxnn = std::max(maxnn, hinst[(size_t)tv[k]].N * hinst[(size_t)tv[k]].N);
+        }
+        g.shmem = nt ? 0 : (size_t)maxnn * sizeof(double2);
+        if (!nt && (int)g.shmem > ctx->lds_per_block) return fail(EH_ESHAPE, "state tile exceeds LDS");
+        im->groups.push_back(g);
+    }
+
+    int rc;
+    std::vector<InstDev> hi = hinst;
+    if ((rc = im->inst.upload(hi)) || (rc = im->tile_inst.upload(t_inst)) || (rc = im->tile_ado0.upload(t_ado0)) ||
+        (rc = im->tile_nado.upload(t_nado)) || (rc = im->link_ptr.upload(lptr)) || (rc = im->link_nbr.upload(lnbr)) ||
+        (rc = im->link_site.upload(lsite)) || (rc = im->pidx.upload(hpidx)) || (rc = im->vreal.upload(hvreal)) ||
+        (rc = im->H.upload(hH)) || (rc = im->Xp.upload(hXp)) || (rc = im->field.upload(hfield)) ||
+        (rc = im->R.upload(hR)) || (rc = im->wr.upload(hwr)) || (rc = im->wc.upload(hwc)) ||
+        (rc = im->damp.upload(hdamp)) || (rc = im->ado0_off.upload(hado0)))
+        return rc;
+    if ((rc = im->step_ctr.alloc(1)) || (rc = im->V.alloc(hH.size())) || (rc = im->P.alloc((size_t)p_off)) ||
+        (rc = im->Y.alloc((size_t)state_off)) || (rc = im->ACC.alloc((size_t)state_off)) ||
+        (rc = im->S1.alloc((size_t)state_off)) || (rc = im->S2.alloc((size_t)state_off)))
+        return rc;
+    HIPCHK(hipStreamCreateWithFlags(&im->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&im->ev0));
+    HIPCHK(hipEventCreate(&im->ev1));
+
+    // capture `chunk` steps into one graph
+    const int per_step = 1 + 4 * (int)im->groups.size();
+    b->chunk = std::max(1, std::min(b->max_steps, 256 / per_step));
+    im->nchunks = (b->max_steps + b->chunk - 1) / b->chunk;
+    StageArgs sa;
+    sa.inst = im->inst.p;
+    sa.tile_inst = im->tile_inst.p;
+    sa.tile_ado0 = im->tile_ado0.p;
+    sa.tile_nado = im->tile_nado.p;
+    sa.step_ctr = im->step_ctr.p;
+    sa.vreal = im->vreal.p;
+    sa.V = im->V.p;
+    sa.link_ptr = im->link_ptr.p;
+    sa.link_nbr = im->link_nbr.p;
+    sa.link_site = im->link_site.p;
+    sa.link_wr = im->wr.p;
+    sa.link_wc = im->wc.p;
+    sa.damp = im->damp.p;
+    sa.y = im->Y.p;
+    sa.acc = im->ACC.p;
+    PrologueArgs pa;
+    pa.inst = im->inst.p;
+    pa.step_ctr = im->step_ctr.p;
+    pa.H = im->H.p;
+    pa.Xp = im->Xp.p;
+    pa.field = im->field.p;
+    pa.pidx = im->pidx.p;
+    pa.R = im->R.p;
+    pa.y = im->Y.p;
+    pa.V = im->V.p;
+    pa.P = im->P.p;
+    HIPCHK(hipStreamBeginCapture(im->stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < b->chunk; k++) {
+        pa.step_off = k;
+        hipLaunchKernelGGL(prologue, dim3(n), dim3(256), 0, im->stream, pa);
+        for (int st = 0; st < 4; st++) {
+            StageArgs a = sa;
+            a.step_off = k;
+            a.stage = st;
+            a.rin = st == 0 ? im->Y.p : (st == 2 ? im->S2.p : im->S1.p);
+            a.out = st == 3 ? im->Y.p : (st == 1 ? im->S2.p : im->S1.p);
+            for (const Group& g : im->groups) {
+                rc = launch_stage(g, a, g.tile_begin, im->stream);
+                if (rc) {
+                    hipGraph_t junk;
+                    (void)hipStreamEndCapture(im->stream, &junk);
+                    return rc;
+                }
+            }
+        }
+    }
+    hipLaunchKernelGGL(tick, dim3(1), dim3(1), 0, im->stream, im->step_ctr.p, b->chunk);
+    HIPCHK(hipStreamEndCapture(im->stream, &im->graph));
+    HIPCHK(hipGraphInstantiate(&im->exec, im->graph, nullptr, nullptr, 0));
+    b->impl = im.release();
+    *out = b.release();
+    return EH_OK;
+}
+
+void batch_destroy(Batch* b) {
+    if (!b) return;
+    if (b->ctx) (void)hipSetDevice(b->ctx->device);
+    if (b->impl) {
+        if (b->impl->stream) (void)hipStreamSynchronize(b->impl->stream);
+        delete b->impl;
+    }
+    delete b;
+}
+
+int batch_run(Batch* b) {
+    if (!b || !b->impl) return fail(EH_EINVAL, "batch_run: null batch");
+    BatchImpl* im = b->impl;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    const long long total = b->total_elems;
+    const int blocks = (int)std::min<long long>(std::max<long long>(1, (total + 255) / 256), 65536);
+    hipLaunchKernelGGL(reset_state, dim3(blocks), dim3(256), 0, im->stream, im->Y.p, total, im->ado0_off.p,
+                       b->n_inst, im->step_ctr.p);
+    hipLaunchKernelGGL(set_ground, dim3((b->n_inst + 255) / 256), dim3(256), 0, im->stream, im->Y.p,
+                       im->ado0_off.p, b->n_inst);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(im->ev0, im->stream));
+    for (int c = 0; c < im->nchunks; c++) HIPCHK(hipGraphLaunch(im->exec, im->stream));
+    HIPCHK(hipEventRecord(im->ev1, im->stream));
+    im->ran = true;
+    b->synced = false;
+    return EH_OK;
+}
+
+int batch_sync(Batch* b) {
+    if (!b || !b->impl) return fail(EH_EINVAL, "batch_sync: null batch");
+    BatchImpl* im = b->impl;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    HIPCHK(hipStreamSynchronize(im->stream));
+    b->P_host.resize((size_t)im->P.n);
+    if (im->P.n) HIPCHK(hipMemcpy(b->P_host.data(), im->P.p, im->P.n * sizeof(double2), hipMemcpyDeviceToHost));
+    b->synced = true;
+    return EH_OK;
+}
+
+int batch_elapsed_ms(Batch* b, double* ms) {
+    if (!b || !b->impl || !b->impl->ran) return fail(EH_EINVAL, "batch has not run");
+    float f = 0;
+    HIPCHK(hipEventElapsedTime(&f, b->impl->ev0, b->impl->ev1));
+    *ms = f;
+    return EH_OK;
+}
+
+// Per-kernel timing pass: `nsteps` steps launched one kernel at a time (no graph)
+// with HIP events around every launch on the batch stream.  out[0] = mean
+// prologue duration, out[1..4] = mean duration of stage 0..3 (all groups of a
+// stage together), in ms.  Leaves the state advanced by nsteps steps.
+int batch_profile(Batch* b, int nsteps, double* out) {
+    if (!b || !b->impl || nsteps <= 0 || !out) return fail(EH_EINVAL, "batch_profile: bad arguments");
+    BatchImpl* im = b->impl;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    const long long total = b->total_elems;
+    const int blocks = (int)std::min<long long>(std::max<long long>(1, (total + 255) / 256), 65536);
+    hipLaunchKernelGGL(reset_state, dim3(blocks), dim3(256), 0, im->stream, im->Y.p, total, im->ado0_off.p,
+                       b->n_inst, im->step_ctr.p);
+    hipLaunchKernelGGL(set_ground, dim3((b->n_inst + 255) / 256), dim3(256), 0, im->stream, im->Y.p,
+                       im->ado0_off.p, b->n_inst);
+    std::vector<hipEvent_t> ev((size_t)6 * nsteps);
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    StageArgs sa;
+    std::memset(&sa, 0, sizeof(sa));
+    sa.inst = im->inst.p;
+    sa.tile_inst = im->tile_inst.p;
+    sa.tile_ado0 = im->tile_ado0.p;
+    sa.tile_nado = im->tile_nado.p;
+    sa.step_ctr = im->step_ctr.p;
+    sa.vreal = im->vreal.p;
+    sa.V = im->V.p;
+    sa.link_ptr = im->link_ptr.p;
+    sa.link_nbr = im->link_nbr.p;
+    sa.link_site = im->link_site.p;
+    sa.link_wr = im->wr.p;
+    sa.link_wc = im->wc.p;
+    sa.damp = im->damp.p;
+    sa.y = im->Y.p;
+    sa.acc = im->ACC.p;
+    PrologueArgs pa;
+    pa.inst = im->inst.p;
+    pa.step_ctr = im->step_ctr.p;
+    pa.H = im->H.p;
+    pa.Xp = im->Xp.p;
+    pa.field = im->field.p;
+    pa.pidx = im->pidx.p;
+    pa.R = im->R.p;
+    pa.y = im->Y.p;
+    pa.V = im->V.p;
+    pa.P = im->P.p;
+    pa.step_off = 0;
+    int rc = EH_OK;
+    for (int k = 0; k < nsteps && rc == EH_OK; k++) {
+        hipEvent_t* e = ev.data() + (size_t)6 * k;
+        (void)hipEventRecord(e[0], im->stream);
+        hipLaunchKernelGGL(prologue, dim3(b->n_inst), dim3(256), 0, im->stream, pa);
+        (void)hipEventRecord(e[1], im->stream);
+        for (int st = 0; st < 4 && rc == EH_OK; st++) {
+            StageArgs a = sa;
+            a.step_off = 0;
+            a.stage = st;
+            a.rin = st == 0 ? im->Y.p : (st == 2 ? im->S2.p : im->S1.p);
+            a.out = st == 3 ? im->Y.p : (st == 1 ? im->S2.p : im->S1.p);
+            for (const Group& g : im->groups)
+                if ((rc = launch_stage(g, a, g.tile_begin, im->stream))) break;
+            (void)hipEventRecord(e[2 + st], im->stream);
+        }
+        hipLaunchKernelGGL(tick, dim3(1), dim3(1), 0, im->stream, im->step_ctr.p, 1);
+    }
+    hipError_t se = hipStreamSynchronize(im->stream);
+    for (int q = 0; q < 5; q++) out[q] = 0.0;
+    if (rc == EH_OK && se == hipSuccess) {
+        for (int k = 0; k < nsteps; k++) {
+            hipEvent_t* e = ev.data() + (size_t)6 * k;
+            for (int q = 0; q < 5; q++) {
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, e[q], e[q + 1]);
+                out[q] += ms / nsteps;
+            }
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (rc) return rc;
+    HIPCHK(se);
+    b->synced = false;
+    return EH_OK;
+}
+
+int batch_get_rho(Batch* b, int i, double* rho_out) {
+    if (!b || i < 0 || i >= b->n_inst || !rho_out) return fail(EH_EINVAL, "batch_get_rho: bad arguments");
+    const Batch::Info& f = b->info[(size_t)i];
+    HIPCHK(hipSetDevice(b->ctx->device));
+    HIPCHK(hipStreamSynchronize(b->impl->stream));
+    HIPCHK(hipMemcpy(rho_out, b->impl->Y.p + f.state_off, (size_t)f.M * f.N * f.N * sizeof(double2),
+                     hipMemcpyDeviceToHost));
+    return EH_OK;
+}
+
+}  // namespace eh

@@ -1,0 +1,46 @@
+// engine.hpp -- host-side interface of the HIP batch engine (engine.hip).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "host.hpp"
+
+struct eh_ctx {
+    int device = 0;
+    char name[256] = {0};
+    int n_cu = 0;
+    int lds_per_block = 0;
+};
+
+namespace eh {
+
+struct BatchImpl;
+
+// Everything one batched run needs, built once and replayed.
+struct Batch {
+    eh_ctx* ctx = nullptr;
+    BatchImpl* impl = nullptr;
+    int n_inst = 0;
+    struct Info {
+        int N = 0, M = 0, nsteps = 0, npoints = 0, nread = 0, mode = 0;
+        int64_t state_off = 0, p_off = 0;
+    };
+    std::vector<Info> info;
+    int64_t total_ados = 0, total_elems = 0;
+    int max_steps = 0, chunk = 0;
+    int64_t ado_updates = 0;
+    std::vector<cplx> P_host;   // readout copy after sync
+    bool synced = false;
+};
+
+int batch_create(eh_ctx* ctx, int n, const eh_instance* inst, Batch** out);
+void batch_destroy(Batch* b);
+int batch_run(Batch* b);
+int batch_sync(Batch* b);
+int batch_elapsed_ms(Batch* b, double* ms);
+int batch_get_rho(Batch* b, int i, double* rho_out);
+int batch_profile(Batch* b, int nsteps, double* out5);
+int ctx_init(int device, eh_ctx* ctx);
+
+}  // namespace eh

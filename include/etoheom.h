@@ -1,0 +1,165 @@
+/*
+ * etoheom.h -- C ABI of the MI355X-native ETO-HEOM propagator (libetoheom.so).
+ *
+ * The reference exposes no FFI for this path; its library boundary is a pair of
+ * C++ free functions over a mutable `param` object:
+ *   void twoD_spectrum_solver(param&, gsl_matrix_complex* H, int nv1, int nv2, int nv3,
+ *                             vector<gsl_complex>& polarization)   etoheom-cpu/lib/dynamics.h:23
+ *   void dynamics_solver(param&, gsl_matrix_complex* H, vector<int>& sites,
+ *                        vector<vector<double>>& population)       etoheom-cpu/lib/dynamics.h:21
+ * plus the key-file parser param::param_2des / param::param_dynamics (param.cpp:18-348),
+ * the hierarchy builder construct_ADO_set (utilize.cpp:26-36), construct_Hal
+ * (dynamics.cpp:3-51) and the dipole assembly of polar.cpp:17-101.  Each entry point
+ * below names the one it replaces.
+ *
+ * Conventions: complex numbers are interleaved (re, im) doubles; matrices are
+ * row-major N*N (the gsl_matrix_complex layout); a hierarchy state is M such
+ * matrices in the reference DFS ADO order.  Every function returns EH_OK (0) or a
+ * negative EH_E* code and never exits the process; eh_last_error() gives the
+ * message of the last failure on the calling thread.  Contexts are not shared
+ * between threads; distinct contexts may be used concurrently.
+ */
+#ifndef ETOHEOM_H
+#define ETOHEOM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EH_OK 0
+#define EH_EINVAL (-1)   /* bad argument (NULL, size, |alpha| = 0, ...) */
+#define EH_EPARSE (-2)   /* key file missing or malformed */
+#define EH_ESHAPE (-3)   /* inconsistent sizes */
+#define EH_ENAN (-4)     /* non-finite state produced */
+#define EH_EHIP (-5)     /* HIP runtime error / no device */
+#define EH_ENOMEM (-6)   /* allocation failed */
+#define EH_ERANGE (-7)   /* index out of range, output capacity too small */
+#define EH_EPULSE (-8)   /* PULSE block with a count other than 3 (param.cpp:227-230) */
+
+#define EH_MODE_2DES 0      /* param_2des: N = n + (n-1)(n-2)/2, S_j = |j+1><j+1| */
+#define EH_MODE_DYNAMICS 1  /* param_dynamics: N = n, S_j = |j><j| */
+
+typedef struct eh_ctx eh_ctx;
+typedef struct eh_problem eh_problem;
+typedef struct eh_batch eh_batch;
+
+/* Plain-data view of one propagation problem (the numbers `param` carries,
+ * param.h:38-82).  Pointers are borrowed for the duration of a call only. */
+typedef struct eh_system {
+    int N;                 /* sys_size */
+    int K;                 /* number of baths (sites) */
+    int Km;                /* modes per bath */
+    int L;                 /* hierarchy depth */
+    int site_offset;       /* 1 (2DES) or 0 (dynamics): bath j couples |j+site_offset> */
+    int reserved;
+    const double* alpha;   /* K*Km complex, [j*Km + k] */
+    const double* alpha_t; /* K*Km complex */
+    const double* gamma;   /* K*Km complex */
+    double pulse[3][4];    /* E0 [fs^-1], tau0 [fs], taup [fs], w0 [fs^-1] */
+    float t_start, t_end, step_size, print_step;   /* float, as param.h:75-78 */
+} eh_system;
+
+/* ---- errors ------------------------------------------------------------ */
+const char* eh_last_error(void);
+
+/* ---- key files (param::param_2des / param_dynamics, param.cpp:18-348) ---- */
+int eh_problem_from_key(const char* path, int mode, eh_problem** out);
+int eh_problem_destroy(eh_problem* p);
+/* sizes and scalars: out[0..] = {mode, single_size, N, K, Km, L, M, n_sample, seed} */
+int eh_problem_dims(const eh_problem* p, int64_t* out9);
+/* fill *sys with a view into p (valid while p lives) */
+int eh_problem_system(const eh_problem* p, eh_system* sys);
+/* named arrays: "Hal" (N*N complex, lambda-shifted), "Disorder" (N*N real sigma/h_bar),
+ * "X_x","X_y","X_z" (N*N complex), "X" (4*N*N complex, after eh_problem_set_polar),
+ * "angle" (4 radians), "lambda" (K), "beta" (1).  Returns the number of doubles
+ * written, or a negative code (EH_ERANGE if cap is too small). */
+int64_t eh_problem_get(const eh_problem* p, const char* name, double* out, int64_t cap);
+/* polar_mat_set (polar.cpp:17-29): polar[3][4] from the POLARIZATION angles */
+int eh_problem_polar_default(const eh_problem* p, double* polar_3x4);
+/* compute_pulse_interaction (polar.cpp:74-101): X[i] = sum_c polar[c][i] X_c */
+int eh_problem_set_polar(eh_problem* p, const double* polar_3x4);
+/* random rotation of polar (polar.cpp:31-72) given the two unit directions c and a */
+int eh_polar_rotate(const double* c3, const double* a3, double* polar_3x4);
+/* construct_Hal (dynamics.cpp:3-51) with the disorder draw dH (single_size^2
+ * real, NULL = none): H = Hal + dH, one-exciton diagonal sorted ascending,
+ * double-exciton block rebuilt.  H_out is N*N complex. */
+int eh_construct_hal(const eh_problem* p, const double* dH, double* H_out);
+
+/* The reference sample loop's random draws (CPU_2DES.cpp:50-60): per sample the
+ * realised Hamiltonian (construct_Hal with gsl_ran_gaussian disorder, N*N complex)
+ * and dipole operators (polar_mat_set + polar_mat_ranrot + compute_pulse_interaction,
+ * 4*N*N complex), from a restatement of GSL's mt19937 / gaussian / dir_3d. */
+int eh_problem_samples(const eh_problem* p, int n_sample, double* H_out, double* X_out);
+/* first `count` raw 32-bit outputs of that generator for `seed` */
+int eh_rng_stream(unsigned long seed, int count, uint32_t* out);
+
+/* ---- hierarchy (Build_ADO / Build_ADO_map, utilize.cpp:3-24) -------------- */
+int64_t eh_hierarchy_size(int d, int L);
+/* multi-indices in DFS order (M*d bytes); up[M*d] / dn[M*d]: index of n+e_p / n-e_p or -1 */
+int eh_hierarchy_build(int d, int L, uint8_t* ado_out, int32_t* up_out, int32_t* dn_out);
+
+/* ---- GPU context ---------------------------------------------------------- */
+int eh_ctx_create(int device, eh_ctx** out);
+int eh_ctx_destroy(eh_ctx* ctx);
+/* device name / CU count / LDS per block, for logs */
+int eh_ctx_info(const eh_ctx* ctx, char* name, int cap, int* n_cu, int* lds_per_block);
+
+/* ---- one propagation (twoD_spectrum_solver, dynamics.cpp:298-432) ---------
+ * X4 = X[0..3] (4*N*N complex).  P_out receives one complex per readout point
+ * (t >= tau3 and i % print_step == 0, read before step i).  rho_out: final state
+ * M*N*N complex or NULL. */
+int eh_2des_solve(eh_ctx* ctx, const eh_system* sys, const double* H, const double* X4,
+                  int nv1, int nv2, int nv3, double* P_out, int64_t cap, int64_t* n_points,
+                  double* rho_out);
+/* dynamics_solver (dynamics.cpp:202-295): pop_out[s*cap + t] = Re rho_0(site_s-1, site_s-1) */
+int eh_dynamics_solve(eh_ctx* ctx, const eh_system* sys, const double* H, const int* sites,
+                      int n_sites, double* pop_out, int64_t cap, int64_t* n_points,
+                      double* rho_out);
+
+/* ---- whole drivers ------------------------------------------------------
+ * main of CPU_2DES (etoheom-cpu/src/CPU_2DES.cpp:14-96): n_sample samples, three
+ * propagations (1,1,1), (1,1,0), (1,0,1) each, P = mean_s(p111 - p110 - p101);
+ * all 3*n_sample propagations run as one batch. */
+int eh_2des_key_run(eh_ctx* ctx, const char* path, double* P_out, int64_t cap, int64_t* n_points);
+/* main of CPU_dynamics (etoheom-cpu/src/CPU_dynamics.cpp:14-49) for the given sites;
+ * n_points is the reference's population length (t_end - t_start)/print_step + 1
+ * evaluated in float, readouts beyond it are dropped and missing ones stay 0 */
+int eh_dynamics_key_run(eh_ctx* ctx, const char* path, const int* sites, int n_sites, double* pop_out,
+                        int64_t cap, int64_t* n_points);
+
+/* ---- batches of independent propagations (replaces the reference's
+ * one-process-per-(tau,T) farming, script/cpu-script/submit_jobs.sh:49-66) -- */
+typedef struct eh_instance {
+    const eh_system* sys;
+    const double* H;       /* N*N complex */
+    const double* X4;      /* 4*N*N complex (2DES) or NULL (dynamics) */
+    int nv[3];
+    int mode;              /* EH_MODE_2DES or EH_MODE_DYNAMICS */
+    const int* sites;      /* dynamics readout sites (1-based) */
+    int n_sites;
+} eh_instance;
+
+int eh_batch_create(eh_ctx* ctx, int n, const eh_instance* inst, eh_batch** out);
+int eh_batch_destroy(eh_batch* b);
+/* reset every state to rho_0 = |0><0| and run all steps (asynchronous) */
+int eh_batch_run(eh_batch* b);
+int eh_batch_sync(eh_batch* b);
+/* elapsed GPU time of the last run, from HIP events on the batch stream */
+int eh_batch_elapsed_ms(eh_batch* b, double* ms);
+/* readouts of instance i: 2DES -> complex per point; dynamics -> n_sites x points real */
+int eh_batch_get_output(eh_batch* b, int i, double* out, int64_t cap, int64_t* n_points);
+int eh_batch_get_rho(eh_batch* b, int i, double* rho_out);
+/* sizes: out[0..] = {n_inst, total ADOs, total state elements, max steps, graph chunk} */
+int eh_batch_dims(const eh_batch* b, int64_t* out5);
+/* per-kernel timing pass: nsteps steps launched without the graph, HIP events
+ * around each launch on the batch stream; out5 = mean ms of {prologue, stage 0..3} */
+int eh_batch_profile(eh_batch* b, int nsteps, double* out5);
+/* ADO-updates per run (sum over instances of M * steps) */
+int64_t eh_batch_ado_updates(const eh_batch* b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
