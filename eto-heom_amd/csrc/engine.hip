@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -85,30 +86,24 @@ __device__ __forceinline__ double2 cmad(double2 acc, double2 a, double2 b) {
 // RHS value k at global element gi -> RK4 update for this stage
 __device__ __forceinline__ void rk_update(const StageArgs& A, const InstDev& I, size_t gi, double2 rin_val,
                                           double kr, double ki) {
-    switch (A.stage) {
-        case 0: {
-            const double2 y = rin_val;
-            A.acc[gi] = make_double2(fma(I.hb[0], kr, y.x), fma(I.hb[0], ki, y.y));
-            A.out[gi] = make_double2(fma(I.h2, kr, y.x), fma(I.h2, ki, y.y));
-            break;
-        }
-        case 1:
-        case 2: {
-            const double2 y = A.y[gi];
-            double2 a = A.acc[gi];
-            const double hb = I.hb[A.stage];
-            const double hs = A.stage == 1 ? I.h2 : I.h;
-            a.x = fma(hb, kr, a.x);
-            a.y = fma(hb, ki, a.y);
-            A.acc[gi] = a;
-            A.out[gi] = make_double2(fma(hs, kr, y.x), fma(hs, ki, y.y));
-            break;
-        }
-        default: {
-            const double2 a = A.acc[gi];
-            A.out[gi] = make_double2(fma(I.hb[3], kr, a.x), fma(I.hb[3], ki, a.y));
-            break;
-        }
+    // stage coefficients without dynamic indexing (keeps InstDev in registers)
+    const int st = A.stage;
+    const double hb = st == 0 ? I.hb[0] : (st == 1 ? I.hb[1] : (st == 2 ? I.hb[2] : I.hb[3]));
+    if (st == 0) {
+        const double2 y = rin_val;
+        A.acc[gi] = make_double2(fma(hb, kr, y.x), fma(hb, ki, y.y));
+        A.out[gi] = make_double2(fma(I.h2, kr, y.x), fma(I.h2, ki, y.y));
+    } else if (st < 3) {
+        const double2 y = A.y[gi];
+        double2 a = A.acc[gi];
+        const double hs = st == 1 ? I.h2 : I.h;
+        a.x = fma(hb, kr, a.x);
+        a.y = fma(hb, ki, a.y);
+        A.acc[gi] = a;
+        A.out[gi] = make_double2(fma(hs, kr, y.x), fma(hs, ki, y.y));
+    } else {
+        const double2 a = A.acc[gi];
+        A.out[gi] = make_double2(fma(hb, kr, a.x), fma(hb, ki, a.y));
     }
 }
 
@@ -141,7 +136,7 @@ __global__ __launch_bounds__(256) void stage_small(StageArgs A) {
     __shared__ double2 sR[G * NN];
     __shared__ double2 sV[NN];
     const int tile = blockIdx.x;
-    const InstDev I = A.inst[A.tile_inst[tile]];
+    const InstDev& I = A.inst[A.tile_inst[tile]];
     const int step = *A.step_ctr + A.step_off;
     if (step >= I.nsteps) return;
     const int a0 = A.tile_ado0[tile], na = A.tile_nado[tile];
@@ -189,7 +184,7 @@ __global__ __launch_bounds__(256) void stage_small(StageArgs A) {
 __global__ __launch_bounds__(256) void stage_generic(StageArgs A) {
     extern __shared__ double2 sR[];
     const int tile = blockIdx.x;
-    const InstDev I = A.inst[A.tile_inst[tile]];
+    const InstDev& I = A.inst[A.tile_inst[tile]];
     const int step = *A.step_ctr + A.step_off;
     if (step >= I.nsteps) return;
     const int N = I.N, NN = N * N;
@@ -230,6 +225,105 @@ __global__ __launch_bounds__(256) void stage_generic(StageArgs A) {
     }
 }
 
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ dbl4 mfma4(double a, double b, dbl4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// N in (16, NP]: one ADO per workgroup, the commutator on FP64 MFMA 16x16x4 tiles.
+// rho_n (and V when VLDS) are staged in LDS as complex, zero-padded to NP x NP
+// with row stride NP+1 (conflict-free for both operand patterns).  Each of the
+// four waves owns (NP/16)^2/4 output tiles and accumulates
+//   C = V rho - rho V  as  Re: Vr.rr - Vi.ri - rr.Vr + ri.Vi,  Im: Vr.ri + Vi.rr - rr.Vi - ri.Vr
+// (the Vi terms only in steps with an active pulse).  MFMA operand layout:
+// A[i][k] at lane i + 16k, B[k][j] at lane j + 16k, D[lane/16 + 4r][lane%16] in reg r.
+template <int NP, bool VLDS>
+__global__ __launch_bounds__(256) void stage_mfma(StageArgs A) {
+    constexpr int LD = NP + 1;
+    constexpr int NB = NP / 16;
+    constexpr int TPW = NB * NB / 4;
+    extern __shared__ double2 smem[];
+    double2* sR = smem;
+    double2* sV = smem + NP * LD;
+    const int tile = blockIdx.x;
+    const InstDev& I = A.inst[A.tile_inst[tile]];
+    const int step = *A.step_ctr + A.step_off;
+    if (step >= I.nsteps) return;
+    const int N = I.N, NN = N * N;
+    const int a0 = A.tile_ado0[tile];
+    const long long base = I.state_off + (long long)a0 * NN;
+    const double2* V = A.V + I.v_off;
+    for (int e = threadIdx.x; e < NP * NP; e += blockDim.x) {
+        const int r = e / NP, c = e - r * NP;
+        double2 x = make_double2(0.0, 0.0), v = make_double2(0.0, 0.0);
+        if (r < N && c < N) {
+            x = A.rin[base + r * N + c];
+            if (VLDS) v = V[r * N + c];
+        }
+        sR[r * LD + c] = x;
+        if (VLDS) sV[r * LD + c] = v;
+    }
+    __syncthreads();
+    const bool vreal = A.vreal[I.step_off + step] != 0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int gado = I.ado_off + a0;
+    const double2 g = A.damp[gado];
+    auto vload = [&](int r, int c) -> double2 {
+        if (VLDS) return sV[r * LD + c];
+        return (r < N && c < N) ? V[r * N + c] : make_double2(0.0, 0.0);
+    };
+#pragma unroll
+    for (int q = 0; q < TPW; ++q) {
+        const int t = wave * TPW + q;
+        const int ti = t / NB, tj = t - (t / NB) * NB;
+        const int ar = ti * 16 + li;     // A-operand row
+        const int bc = tj * 16 + li;     // B-operand column
+        dbl4 cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+        if (vreal) {
+#pragma unroll 4
+            for (int k0 = 0; k0 < NP; k0 += 4) {
+                const int kk = k0 + lk;
+                const double va = vload(ar, kk).x, vb = vload(kk, bc).x;
+                const double2 rb = sR[kk * LD + bc], ra = sR[ar * LD + kk];
+                cr = mfma4(va, rb.x, cr);
+                ci = mfma4(va, rb.y, ci);
+                cr = mfma4(ra.x, -vb, cr);
+                ci = mfma4(ra.y, -vb, ci);
+            }
+        } else {
+#pragma unroll 2
+            for (int k0 = 0; k0 < NP; k0 += 4) {
+                const int kk = k0 + lk;
+                const double2 va = vload(ar, kk), vb = vload(kk, bc);
+                const double2 rb = sR[kk * LD + bc], ra = sR[ar * LD + kk];
+                cr = mfma4(va.x, rb.x, cr);
+                ci = mfma4(va.x, rb.y, ci);
+                cr = mfma4(-va.y, rb.y, cr);
+                ci = mfma4(va.y, rb.x, ci);
+                cr = mfma4(ra.x, -vb.x, cr);
+                ci = mfma4(ra.y, -vb.x, ci);
+                cr = mfma4(ra.y, vb.y, cr);
+                ci = mfma4(ra.x, -vb.y, ci);
+            }
+        }
+        const int col = tj * 16 + li;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = ti * 16 + lk + 4 * r;
+            if (row < N && col < N) {
+                double dr = ci[r], di = -cr[r];   // -i * C
+                const double2 x = sR[row * LD + col];
+                dr = fma(g.x, x.x, fma(-g.y, x.y, dr));
+                di = fma(g.x, x.y, fma(g.y, x.x, di));
+                add_links(A, I, gado, row, col, row * N + col, NN, dr, di);
+                rk_update(A, I, (size_t)(base + row * N + col), x, dr, di);
+            }
+        }
+    }
+}
+
 struct PrologueArgs {
     const InstDev* inst;
     const int* step_ctr;
@@ -247,7 +341,7 @@ struct PrologueArgs {
 // V(t_i) = H - e_1 X[0] - e_2 X[1]^T - e_3 X[2]^T  (dynamics.cpp:348-363), and
 // the readouts Tr(rho_0 R_r) before step i (dynamics.cpp:367-381, :246-253)
 __global__ __launch_bounds__(256) void prologue(PrologueArgs A) {
-    const InstDev I = A.inst[blockIdx.x];
+    const InstDev& I = A.inst[blockIdx.x];
     const int step = *A.step_ctr + A.step_off;
     if (step >= I.nsteps) return;
     const int N = I.N, NN = N * N;
@@ -409,6 +503,8 @@ static int launch_stage(const Group& g, const StageArgs& base, int tile0, hipStr
         EH_CASE(1) EH_CASE(2) EH_CASE(3) EH_CASE(4) EH_CASE(5) EH_CASE(6) EH_CASE(7) EH_CASE(8)
         EH_CASE(9) EH_CASE(10) EH_CASE(11) EH_CASE(12) EH_CASE(13) EH_CASE(14) EH_CASE(15) EH_CASE(16)
 #undef EH_CASE
+        case -32: hipLaunchKernelGGL((stage_mfma<32, true>), grid, block, g.shmem, s, a); break;
+        case -64: hipLaunchKernelGGL((stage_mfma<64, false>), grid, block, g.shmem, s, a); break;
         default: hipLaunchKernelGGL(stage_generic, grid, block, g.shmem, s, a); break;
     }
     HIPCHK(hipGetLastError());
@@ -434,7 +530,9 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
     int64_t state_off = 0, p_off = 0;
     int ado_off = 0, v_off = 0, r_off = 0, step_off = 0;
     // tiles grouped by kernel variant
-    std::vector<std::vector<int>> tiles_by_nt(17);   // each entry: triples (inst, ado0, nado)
+    // kernel variants: 1..16 stage_small<N>, 17 = stage_mfma<32>, 18 = stage_mfma<64>, 0 = stage_generic
+    std::vector<std::vector<int>> tiles_by_nt(19);   // each entry: triples (inst, ado0, nado)
+    const bool use_mfma = !(std::getenv("ETOHEOM_NO_MFMA") && std::getenv("ETOHEOM_NO_MFMA")[0] == '1');
 
     for (int i = 0; i < n; i++) {
         const eh_instance& q = in[i];
@@ -531,7 +629,7 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
         b->ado_updates += (int64_t)M * s.nsteps;
         b->max_steps = std::max(b->max_steps, s.nsteps);
         // tiles
-        const int nt = N <= 16 ? N : 0;
+        const int nt = N <= 16 ? N : (!use_mfma ? 0 : (N <= 32 ? 17 : 18));
         const int G = nt ? (NN >= 512 ? 1 : 512 / NN) : 1;
         for (int a0 = 0; a0 < M; a0 += G) {
             auto& tv = tiles_by_nt[(size_t)nt];
@@ -548,11 +646,11 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
     }
     b->total_ados = ado_off;
     b->total_elems = state_off;
-    for (int nt = 0; nt <= 16; nt++) {
+    for (int nt = 0; nt <= 18; nt++) {
         const auto& tv = tiles_by_nt[(size_t)nt];
         if (tv.empty()) continue;
         Group g;
-        g.nt = nt;
+        g.nt = nt <= 16 ? nt : (nt == 17 ? -32 : -64);
         g.tile_begin = (int)t_inst.size();
         g.tile_count = (int)(tv.size() / 3);
         int maxnn = 0;
@@ -562,8 +660,20 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
             t_nado.push_back(tv[k + 2]);
             maxnn = std::max(maxnn, hinst[(size_t)tv[k]].N * hinst[(size_t)tv[k]].N);
         }
-        g.shmem = nt ? 0 : (size_t)maxnn * sizeof(double2);
-        if (!nt && (int)g.shmem > ctx->lds_per_block) return fail(EH_ESHAPE, "state tile exceeds LDS");
+        if (g.nt == -32)
+            g.shmem = (size_t)2 * 32 * 33 * sizeof(double2);
+        else if (g.nt == -64)
+            g.shmem = (size_t)64 * 65 * sizeof(double2);
+        else
+            g.shmem = nt ? 0 : (size_t)maxnn * sizeof(double2);
+        if ((int)g.shmem > ctx->lds_per_block) return fail(EH_ESHAPE, "state tile exceeds LDS");
+        if (g.shmem > 65536) {
+            hipError_t fe = g.nt == -64 ? hipFuncSetAttribute((const void*)stage_mfma<64, false>,
+                                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)g.shmem)
+                                        : hipFuncSetAttribute((const void*)stage_generic,
+                                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)g.shmem);
+            if (fe != hipSuccess) return fail(EH_ESHAPE, "kernel cannot use the LDS the tile needs");
+        }
         im->groups.push_back(g);
     }
 

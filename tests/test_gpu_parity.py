@@ -78,7 +78,7 @@ def test_batch_of_mixed_sizes_matches_oracle(ctx):
     items = [configs.aggregate(2, 3, nsteps=200, taus=(20.0, 30.0, 40.0)),
              configs.aggregate(3, 2, nsteps=150, taus=(20.0, 35.0, 45.0), sigma=50.0),
              configs.aggregate(4, 2, nsteps=180, taus=(30.0, 20.0, 50.0)),
-             configs.dimer(L=4, nsteps=250)]
+             configs.dimer(L=4, nsteps=250, t1=20.0, t2=20.0)]
     b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)} for p in items])
     b.run()
     b.sync()
@@ -115,3 +115,42 @@ def test_key_driver_matches_sampled_oracle(ctx, tmp_path):
     assert len(P) == len(acc) == 6
     # the subtraction cancels the first-order signal: compare against the scale of p1
     assert np.max(np.abs(P - acc)) / np.max(np.abs(p1)) < TOL
+
+
+def test_chain_n56_mfma64_matches_oracle(ctx):
+    """N = 56 (64-padded MFMA tiles) on a truncated run: L = 1 and 30 steps."""
+    p = configs.chain(L=1, nsteps=30)
+    p.pulses[2, 1] = 0.0
+    P, rho = E.twoD_spectrum_solver(ctx, p, p.H, 1, 1, 1, X=p.X, want_rho=True)
+    Pr, rr = O.twod(oproblem(p), nv=(1, 1, 1), want_rho=True)
+    assert rel(P, Pr) < TOL
+    assert rel(rho, rr) < TOL
+
+
+def test_cli_2des_prints_reference_format(ctx, tmp_path):
+    import subprocess
+    txt = open("tests/golden/cpu_dimer_ohmic.key-tmpl").read()
+    txt = txt.replace("TAU1", "30").replace("TAU2", "50").replace("TAU3", "70").replace("T_END", "120")
+    txt = txt.replace("50 114", "3 114")
+    key = tmp_path / "k.key"
+    key.write_text(txt)
+    out = subprocess.run(["eto-heom_amd/bin/etoheom_2des", str(key)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    P = E.run_2des_key(ctx, str(key))
+    want = "".join(f"{v.real:.6e} {v.imag:.6e}\n" for v in P)
+    assert out.stdout == want and len(P) == 6
+
+
+def test_cli_dynamics_prints_reference_format(ctx):
+    import subprocess
+    out = subprocess.run(["eto-heom_amd/bin/etoheom_dynamics", "tests/golden/key_0_0.key"], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.split("\n")
+    assert lines[0] == "Now is CPU version" and lines[1] == "Running tests/golden/key_0_0.key"
+    assert lines[3] == "site 1 population: " and lines[4] == ""
+    k = E.Param.from_key("tests/golden/key_0_0.key", E.MODE_DYNAMICS)
+    pop = E.dynamics_solver(ctx, k, k.get("Hal"), sites=(1, 2))
+    got1 = [float(x) for x in lines[5:5 + 101]]
+    assert np.allclose(got1, pop[0], rtol=1e-5, atol=1e-12)
+    assert lines[5] == "1" and lines[5 + 101] == "" and lines[5 + 102] == "site 2 population: "
