@@ -23,8 +23,13 @@ sys.path.insert(0, os.path.join(ROOT, "eto-heom_amd"))
 HBM_PEAK_GBS = 8000.0
 
 
-def workload(name):
+def workload(name, nsteps=None):
     from etoheom_amd import configs
+    if nsteps:   # shortened time grid, for counter runs only
+        items, desc = workload(name)
+        for p in items:
+            p.t_end = p.t_start + p.step_size * (nsteps - 1)
+        return items, desc + f" (shortened to {nsteps} RK4 steps)"
     if name == "heptamer":
         return [configs.heptamer()], "heptamer N=29 (ground+7+21), d=14, L=4, 3060 ADOs, 2000 RK4 steps dt=0.5 fs"
     if name == "dimer":
@@ -38,25 +43,39 @@ def workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(p, seconds=20.0):
-    """The oracle (a scalar FP64 port of the reference CPU path) on one core,
-    over whole RK4 steps of the same instance until ~`seconds` have elapsed."""
+def cpu_baseline(p, rho, seconds=20.0):
+    """The oracle (a scalar FP64 port of the reference CPU path, with its dense
+    one-hot GEMMs and hashed neighbour lookups) on one core.  It starts from the
+    GPU's final hierarchy state, so every ADO is dense as in the bulk of a
+    propagation (the first steps, before the pulses, have an all-zero hierarchy
+    that the reference's zero-skipping GEMM passes over), and runs whole RK4
+    steps after the last pulse until ~`seconds` have elapsed."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     q = O.Problem(H=p.H, X=p.X, alpha=p.alpha, alpha_t=p.alpha_t, gamma=p.gamma, L=p.L,
-                  site_offset=p.site_offset, pulses=p.pulses, t_start=p.t_start, t_end=p.t_end,
+                  site_offset=p.site_offset, pulses=p.pulses, t_start=p.t_end, t_end=p.t_end + 10000.0,
                   step_size=p.step_size, print_step=p.print_step)
     M = q.num_ado()
     steps, elapsed = 1, 0.0
     while True:
         t0 = time.perf_counter()
-        O.twod(q, nv=(1, 1, 1), max_steps=steps)
+        O.twod(q, nv=(1, 1, 1), rho_init=rho, max_steps=steps)
         elapsed = time.perf_counter() - t0
         if elapsed > seconds / 3 or steps >= 4096:
             break
         steps *= 4 if elapsed < seconds / 30 else 2
     return {"value": M * steps / elapsed, "unit": "ADO-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} RK4 steps of the same instance ({M} ADOs), {elapsed:.1f} s, 1 thread"}
+            "sample": f"{steps} RK4 steps of the same instance ({M} ADOs) from its dense final state, "
+                      f"{elapsed:.1f} s, 1 thread"}
+
+
+def traffic_of(workload):
+    """HBM bytes per stage launch from the committed rocprofv3 --pmc summary, if any."""
+    f = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
+    if not os.path.exists(f):
+        return None, None
+    d = json.load(open(f))
+    return d["traffic_bytes"], d["source"]
 
 
 def main():
@@ -67,6 +86,9 @@ def main():
     ap.add_argument("--workload", default="heptamer")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--nsteps", type=int, default=0, help="shorten the propagation (counter runs only)")
+    ap.add_argument("--copy-calibrate", action="store_true",
+                    help="also run a plain 2 GiB read + 2 GiB write elementwise copy (counter scale)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); independent propagations shard "
@@ -77,7 +99,14 @@ def main():
     import torch
     import etoheom_amd as E
 
-    items, desc = workload(a.workload)
+    if a.copy_calibrate:
+        x = torch.ones(1 << 28, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        del x, y
+    items, desc = workload(a.workload, a.nsteps)
     ctx = E.Context(0)
     batch = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)} for p in items])
     for _ in range(a.warmup):
@@ -101,6 +130,7 @@ def main():
     bytes_per_launch = 64.0 * elems        # 256 N^2 B per ADO-update over 4 stage launches
     achieved = bytes_per_launch / (stage_ms * 1e-3) / 1e9
 
+    traffic, tsrc = traffic_of(a.workload)
     ups = batch.ado_updates * a.steps
     value = ups / wall
     rec = {
@@ -121,12 +151,12 @@ def main():
                    "grid_points_per_s": len(items) * a.steps / wall / 3.0,
                    "gpu_event_ms_last_run": gpu_ms, "device": ctx.name, "finite": finite},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": tsrc,
                      "kernel": "stage (fused RHS + RK4 stage)", "stage_ms": [float(x) for x in prof[1:5]],
                      "prologue_ms": float(prof[0]), "bytes_per_launch": bytes_per_launch},
     }
     if not a.no_cpu:
-        rec["cpu_baseline"] = cpu_baseline(items[0])
+        rec["cpu_baseline"] = cpu_baseline(items[0], batch.rho(0))
     print(json.dumps(rec))
     return 0
 

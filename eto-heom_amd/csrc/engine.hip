@@ -43,15 +43,18 @@ namespace eh {
 struct InstDev {
     long long state_off;   // complex offset of ADO 0 in the state buffers
     long long p_off;       // complex offset in the readout buffer
+    long long lk_off;      // offset of ADO 0's row in the (ADO, mode) neighbour tables
     int N, M;
-    int ado_off;           // offset in the global ADO numbering (CSR, damping)
+    int ado_off;           // offset in the global ADO numbering (damping)
     int nsteps;
-    int v_off;             // complex offset of this instance's N*N blocks (H, V, X'_q, R_r)
+    int v_off;             // complex offset of this instance's N*N blocks (H, V, X'_q)
     int step_off;          // offset in the per-step tables
     int nread;             // readout operators
     int npoints;
-    int real_h;            // H has no imaginary part
     int r_off;             // complex offset of this instance's readout operators
+    int d, K, Km;          // hierarchy dimension d = K*Km, baths, modes per bath
+    int so;                // site offset: bath j couples state j + so
+    int al_off;            // offset of alpha / alpha~ (K*Km complex)
     double h2, h, hb[4];
 };
 
@@ -65,12 +68,13 @@ struct StageArgs {
     int stage;
     const uint8_t* vreal;    // per step: V(t_i) has no imaginary part
     const double2* V;        // per instance N*N
-    const int* link_ptr;     // per global ADO + 1
-    const int* link_nbr;
-    const int* link_site;
-    const double2* link_wr;
-    const double2* link_wc;
-    const double2* damp;     // per global ADO
+    const int* nb_up;        // per (ADO, p): index of n + e_p or -1
+    const int* nb_dn;        // per (ADO, p): index of n - e_p or -1
+    const double* cx;        // per (ADO, p): sqrt((n_p + 1) |alpha_p|)
+    const double* cy;        // per (ADO, p): sqrt(n_p / |alpha_p|)
+    const double2* alpha;    // per instance K*Km
+    const double2* alpha_t;
+    const double2* damp;     // per global ADO: -sum_p n_p gamma_p
     const double2* rin;      // stage input
     const double2* y;        // Y
     double2* acc;            // ACC
@@ -83,71 +87,134 @@ __device__ __forceinline__ double2 cmad(double2 acc, double2 a, double2 b) {
     return acc;
 }
 
-// RHS value k at global element gi -> RK4 update for this stage
-__device__ __forceinline__ void rk_update(const StageArgs& A, const InstDev& I, size_t gi, double2 rin_val,
-                                          double kr, double ki) {
-    // stage coefficients without dynamic indexing (keeps InstDev in registers)
+__device__ __forceinline__ double stage_hb(const InstDev& I, int st) {
+    return st == 0 ? I.hb[0] : (st == 1 ? I.hb[1] : (st == 2 ? I.hb[2] : I.hb[3]));
+}
+
+// RK4 stage update of one element from its RHS value k, with y / acc already loaded:
+//   stage 0: ACC = y + h/6 k, out = y + h/2 k;  1: ACC += h/3 k, out = y + h/2 k;
+//   2: ACC += h/3 k, out = y + h k;  3: out (= Y) = ACC + h/6 k
+__device__ __forceinline__ void rk_store(const StageArgs& A, const InstDev& I, size_t gi, double2 y, double2 a,
+                                         double kr, double ki) {
     const int st = A.stage;
-    const double hb = st == 0 ? I.hb[0] : (st == 1 ? I.hb[1] : (st == 2 ? I.hb[2] : I.hb[3]));
+    const double hb = stage_hb(I, st);
     if (st == 0) {
-        const double2 y = rin_val;
         A.acc[gi] = make_double2(fma(hb, kr, y.x), fma(hb, ki, y.y));
         A.out[gi] = make_double2(fma(I.h2, kr, y.x), fma(I.h2, ki, y.y));
     } else if (st < 3) {
-        const double2 y = A.y[gi];
-        double2 a = A.acc[gi];
         const double hs = st == 1 ? I.h2 : I.h;
-        a.x = fma(hb, kr, a.x);
-        a.y = fma(hb, ki, a.y);
-        A.acc[gi] = a;
+        A.acc[gi] = make_double2(fma(hb, kr, a.x), fma(hb, ki, a.y));
         A.out[gi] = make_double2(fma(hs, kr, y.x), fma(hs, ki, y.y));
     } else {
-        const double2 a = A.acc[gi];
         A.out[gi] = make_double2(fma(hb, kr, a.x), fma(hb, ki, a.y));
     }
 }
 
-// coupling gathers for element (ra, cb) of ADO gado; adds into (dr, di)
-__device__ __forceinline__ void add_links(const StageArgs& A, const InstDev& I, int gado, int ra, int cb, int ab,
-                                          int NN, double& dr, double& di) {
-    const int l0 = A.link_ptr[gado], l1 = A.link_ptr[gado + 1];
-    for (int l = l0; l < l1; ++l) {
-        const int s = A.link_site[l];
-        const bool rw = (ra == s), cl = (cb == s);
-        if (rw | cl) {
-            double2 w = make_double2(0.0, 0.0);
-            if (rw) w = A.link_wr[l];
-            if (cl) {
-                const double2 c = A.link_wc[l];
-                w.x += c.x;
-                w.y += c.y;
-            }
-            const double2 x = A.rin[I.state_off + (long long)A.link_nbr[l] * NN + ab];
-            dr = fma(w.x, x.x, fma(-w.y, x.y, dr));
-            di = fma(w.x, x.y, fma(w.y, x.x, di));
+// y and acc of element gi as the stage needs them (stage 0 takes y = rin value)
+__device__ __forceinline__ void rk_load(const StageArgs& A, size_t gi, double2 rin_val, double2& y, double2& a) {
+    const int st = A.stage;
+    y = st == 0 ? rin_val : (st < 3 ? A.y[gi] : make_double2(0.0, 0.0));
+    a = st == 0 ? make_double2(0.0, 0.0) : A.acc[gi];
+}
+
+// Coupling gathers of one ADO into LDS.  For bath j (site s = j + so), the
+// reference's one-hot products S_j rho_m and rho_m S_j (dynamics.cpp:143-179)
+// touch only row s and column s, so
+//   cp[(j*2 + 0)*N + b] = sum_k  i x rho_{n+e_p}[s][b] + i alpha y rho_{n-e_p}[s][b]      (row s)
+//   cp[(j*2 + 1)*N + a] = sum_k -i x rho_{n+e_p}[a][s] - i alpha~ y rho_{n-e_p}[a][s]     (column s)
+// with p = j + k K.  Every load is independent, so a thread issues all of its
+// 2 Km gathers before it waits.
+__device__ __forceinline__ void gather_couplings(const StageArgs& A, const InstDev& I, int n0, int na, double2* cp,
+                                                 int tid, int nthr) {
+    const int N = I.N, NN = N * N, K = I.K, Km = I.Km, d = I.d;
+    const int per = 2 * K * N;
+    const int total = per * na;
+    for (int e = tid; e < total; e += nthr) {
+        const int g = e / per, eg = e - g * per;
+        const int j = eg / (2 * N), rem = eg - j * 2 * N;
+        const int rc = rem / N, pos = rem - rc * N;
+        const int s = j + I.so;
+        const int ab = rc == 0 ? s * N + pos : pos * N + s;
+        const long long row0 = I.lk_off + (long long)(n0 + g) * d;
+        double2 acc = make_double2(0.0, 0.0);
+        for (int k = 0; k < Km; ++k) {
+            const long long q = row0 + j + k * K;
+            const int mu = A.nb_up[q], md = A.nb_dn[q];
+            const double2 xu = mu >= 0 ? A.rin[I.state_off + (long long)mu * NN + ab] : make_double2(0.0, 0.0);
+            const double2 xd = md >= 0 ? A.rin[I.state_off + (long long)md * NN + ab] : make_double2(0.0, 0.0);
+            const double x = A.cx[q], y = A.cy[q];
+            const double2 al = rc == 0 ? A.alpha[I.al_off + j * Km + k] : A.alpha_t[I.al_off + j * Km + k];
+            // up:   row  i x,  column -i x;   down: row  i alpha y,  column -i alpha~ y
+            const double sg = rc == 0 ? 1.0 : -1.0;
+            const double2 wu = make_double2(0.0, sg * x);
+            const double2 wd = make_double2(-sg * al.y * y, sg * al.x * y);
+            acc = cmad(acc, wu, xu);
+            acc = cmad(acc, wd, xd);
         }
+        cp[e] = acc;
     }
 }
 
+// coupling term of element (row, col) from the gathered rows/columns
+__device__ __forceinline__ double2 coupling_of(const InstDev& I, const double2* cp, int row, int col) {
+    double2 c = make_double2(0.0, 0.0);
+    const int jr = row - I.so, jc = col - I.so;
+    if (jr >= 0 && jr < I.K) {
+        const double2 v = cp[(jr * 2 + 0) * I.N + col];
+        c.x += v.x;
+        c.y += v.y;
+    }
+    if (jc >= 0 && jc < I.K) {
+        const double2 v = cp[(jc * 2 + 1) * I.N + row];
+        c.x += v.x;
+        c.y += v.y;
+    }
+    return c;
+}
+
+// N <= 16: a tile of G = 512/N^2 ADOs per workgroup, two elements per thread.
 template <int NT>
 __global__ __launch_bounds__(256) void stage_small(StageArgs A) {
     constexpr int NN = NT * NT;
     constexpr int G = NN >= 512 ? 1 : 512 / NN;
+    constexpr int EPT = (G * NN + 255) / 256;
+    constexpr int CPMAX = G * 2 * NT * NT;   // K <= N
     __shared__ double2 sR[G * NN];
     __shared__ double2 sV[NN];
+    __shared__ double2 sC[CPMAX > 0 ? CPMAX : 1];
     const int tile = blockIdx.x;
     const InstDev& I = A.inst[A.tile_inst[tile]];
     const int step = *A.step_ctr + A.step_off;
     if (step >= I.nsteps) return;
     const int a0 = A.tile_ado0[tile], na = A.tile_nado[tile];
-    const double2* V = A.V + I.v_off;
-    for (int e = threadIdx.x; e < NN; e += blockDim.x) sV[e] = V[e];
     const long long base = I.state_off + (long long)a0 * NN;
     const int nel = na * NN;
-    for (int e = threadIdx.x; e < nel; e += blockDim.x) sR[e] = A.rin[base + e];
+    // issue every load of the tile first: own elements, y/acc, V, coupling gathers
+    double2 rv[EPT], yv[EPT], av[EPT];
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+        const int e = threadIdx.x + q * 256;
+        rv[q] = e < nel ? A.rin[base + e] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+        const int e = threadIdx.x + q * 256;
+        if (e < nel) rk_load(A, (size_t)(base + e), rv[q], yv[q], av[q]);
+    }
+    for (int e = threadIdx.x; e < NN; e += 256) sV[e] = A.V[I.v_off + e];
+    const int cpn = 2 * I.K * NT;
+    gather_couplings(A, I, a0, na, sC, threadIdx.x, 256);
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+        const int e = threadIdx.x + q * 256;
+        if (e < nel) sR[e] = rv[q];
+    }
     __syncthreads();
     const bool vreal = A.vreal[I.step_off + step] != 0;
-    for (int e = threadIdx.x; e < nel; e += blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+        const int e = threadIdx.x + q * 256;
+        if (e >= nel) break;
         const int nl = e / NN, ab = e - nl * NN, ra = ab / NT, cb = ab - ra * NT;
         const double2* R = sR + nl * NN;
         double cr = 0.0, ci = 0.0;   // (V rho - rho V)[ra][cb]
@@ -171,31 +238,34 @@ __global__ __launch_bounds__(256) void stage_small(StageArgs A) {
             }
         }
         double dr = ci, di = -cr;   // -i * comm
-        const int gado = I.ado_off + a0 + nl;
-        const double2 g = A.damp[gado], r = R[ab];
+        const double2 g = A.damp[I.ado_off + a0 + nl], r = rv[q];
         dr = fma(g.x, r.x, fma(-g.y, r.y, dr));
         di = fma(g.x, r.y, fma(g.y, r.x, di));
-        add_links(A, I, gado, ra, cb, ab, NN, dr, di);
-        rk_update(A, I, (size_t)(base + e), r, dr, di);
+        const double2 cpl = coupling_of(I, sC + nl * cpn, ra, cb);
+        dr += cpl.x;
+        di += cpl.y;
+        rk_store(A, I, (size_t)(base + e), yv[q], av[q], dr, di);
     }
 }
 
-// any N: one ADO per workgroup, rho_n staged in dynamic LDS, V read through the cache
+// any N <= 64: one ADO per workgroup, rho_n staged in dynamic LDS, V read through the cache
 __global__ __launch_bounds__(256) void stage_generic(StageArgs A) {
-    extern __shared__ double2 sR[];
+    extern __shared__ double2 smem[];
     const int tile = blockIdx.x;
     const InstDev& I = A.inst[A.tile_inst[tile]];
     const int step = *A.step_ctr + A.step_off;
     if (step >= I.nsteps) return;
     const int N = I.N, NN = N * N;
+    double2* sR = smem;
+    double2* sC = smem + NN;
     const int a0 = A.tile_ado0[tile];
     const double2* V = A.V + I.v_off;
     const long long base = I.state_off + (long long)a0 * NN;
     for (int e = threadIdx.x; e < NN; e += blockDim.x) sR[e] = A.rin[base + e];
+    gather_couplings(A, I, a0, 1, sC, threadIdx.x, blockDim.x);
     __syncthreads();
     const bool vreal = A.vreal[I.step_off + step] != 0;
-    const int gado = I.ado_off + a0;
-    const double2 g = A.damp[gado];
+    const double2 g = A.damp[I.ado_off + a0];
     for (int ab = threadIdx.x; ab < NN; ab += blockDim.x) {
         const int ra = ab / N, cb = ab - ra * N;
         double cr = 0.0, ci = 0.0;
@@ -220,8 +290,10 @@ __global__ __launch_bounds__(256) void stage_generic(StageArgs A) {
         const double2 r = sR[ab];
         dr = fma(g.x, r.x, fma(-g.y, r.y, dr));
         di = fma(g.x, r.y, fma(g.y, r.x, di));
-        add_links(A, I, gado, ra, cb, ab, NN, dr, di);
-        rk_update(A, I, (size_t)(base + ab), r, dr, di);
+        const double2 cpl = coupling_of(I, sC, ra, cb);
+        double2 yv, av;
+        rk_load(A, (size_t)(base + ab), r, yv, av);
+        rk_store(A, I, (size_t)(base + ab), yv, av, dr + cpl.x, di + cpl.y);
     }
 }
 
@@ -238,14 +310,18 @@ __device__ __forceinline__ dbl4 mfma4(double a, double b, dbl4 c) {
 //   C = V rho - rho V  as  Re: Vr.rr - Vi.ri - rr.Vr + ri.Vi,  Im: Vr.ri + Vi.rr - rr.Vi - ri.Vr
 // (the Vi terms only in steps with an active pulse).  MFMA operand layout:
 // A[i][k] at lane i + 16k, B[k][j] at lane j + 16k, D[lane/16 + 4r][lane%16] in reg r.
+// All global loads of the workgroup (rho_n, V, coupling gathers, and y / acc of
+// the lane's output elements) are issued before the first wait.
 template <int NP, bool VLDS>
 __global__ __launch_bounds__(256) void stage_mfma(StageArgs A) {
     constexpr int LD = NP + 1;
     constexpr int NB = NP / 16;
     constexpr int TPW = NB * NB / 4;
+    constexpr int SPT = NP * NP / 256;   // staged entries per thread
     extern __shared__ double2 smem[];
     double2* sR = smem;
     double2* sV = smem + NP * LD;
+    double2* sC = smem + (VLDS ? 2 : 1) * NP * LD;
     const int tile = blockIdx.x;
     const InstDev& I = A.inst[A.tile_inst[tile]];
     const int step = *A.step_ctr + A.step_off;
@@ -254,30 +330,55 @@ __global__ __launch_bounds__(256) void stage_mfma(StageArgs A) {
     const int a0 = A.tile_ado0[tile];
     const long long base = I.state_off + (long long)a0 * NN;
     const double2* V = A.V + I.v_off;
-    for (int e = threadIdx.x; e < NP * NP; e += blockDim.x) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+
+    double2 xr[SPT], xv[SPT];
+#pragma unroll
+    for (int q = 0; q < SPT; ++q) {
+        const int e = threadIdx.x + q * 256;
         const int r = e / NP, c = e - r * NP;
-        double2 x = make_double2(0.0, 0.0), v = make_double2(0.0, 0.0);
-        if (r < N && c < N) {
-            x = A.rin[base + r * N + c];
-            if (VLDS) v = V[r * N + c];
+        const bool in = r < N && c < N;
+        xr[q] = in ? A.rin[base + r * N + c] : make_double2(0.0, 0.0);
+        if (VLDS) xv[q] = in ? V[r * N + c] : make_double2(0.0, 0.0);
+    }
+    // y / acc of this lane's output elements
+    double2 yv[TPW][4], av[TPW][4];
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+        const int tt = wave * TPW + t, ti = tt / NB, tj = tt - (tt / NB) * NB;
+        const int col = tj * 16 + li;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = ti * 16 + lk + 4 * r;
+            yv[t][r] = make_double2(0.0, 0.0);
+            av[t][r] = make_double2(0.0, 0.0);
+            if (row < N && col < N && A.stage != 0) {
+                const size_t gi = (size_t)(base + row * N + col);
+                if (A.stage < 3) yv[t][r] = A.y[gi];
+                av[t][r] = A.acc[gi];
+            }
         }
-        sR[r * LD + c] = x;
-        if (VLDS) sV[r * LD + c] = v;
+    }
+    gather_couplings(A, I, a0, 1, sC, threadIdx.x, 256);
+#pragma unroll
+    for (int q = 0; q < SPT; ++q) {
+        const int e = threadIdx.x + q * 256;
+        const int r = e / NP, c = e - r * NP;
+        sR[r * LD + c] = xr[q];
+        if (VLDS) sV[r * LD + c] = xv[q];
     }
     __syncthreads();
     const bool vreal = A.vreal[I.step_off + step] != 0;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int li = lane & 15, lk = lane >> 4;
-    const int gado = I.ado_off + a0;
-    const double2 g = A.damp[gado];
+    const double2 g = A.damp[I.ado_off + a0];
     auto vload = [&](int r, int c) -> double2 {
         if (VLDS) return sV[r * LD + c];
         return (r < N && c < N) ? V[r * N + c] : make_double2(0.0, 0.0);
     };
 #pragma unroll
-    for (int q = 0; q < TPW; ++q) {
-        const int t = wave * TPW + q;
-        const int ti = t / NB, tj = t - (t / NB) * NB;
+    for (int t = 0; t < TPW; ++t) {
+        const int tt = wave * TPW + t;
+        const int ti = tt / NB, tj = tt - (tt / NB) * NB;
         const int ar = ti * 16 + li;     // A-operand row
         const int bc = tj * 16 + li;     // B-operand column
         dbl4 cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
@@ -317,8 +418,9 @@ __global__ __launch_bounds__(256) void stage_mfma(StageArgs A) {
                 const double2 x = sR[row * LD + col];
                 dr = fma(g.x, x.x, fma(-g.y, x.y, dr));
                 di = fma(g.x, x.y, fma(g.y, x.x, di));
-                add_links(A, I, gado, row, col, row * N + col, NN, dr, di);
-                rk_update(A, I, (size_t)(base + row * N + col), x, dr, di);
+                const double2 cpl = coupling_of(I, sC, row, col);
+                const double2 y0 = A.stage == 0 ? x : yv[t][r];
+                rk_store(A, I, (size_t)(base + row * N + col), y0, av[t][r], dr + cpl.x, di + cpl.y);
             }
         }
     }
@@ -457,9 +559,10 @@ struct BatchImpl {
     hipGraph_t graph = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf<InstDev> inst;
-    DevBuf<int> tile_inst, tile_ado0, tile_nado, step_ctr, link_ptr, link_nbr, link_site, pidx;
+    DevBuf<int> tile_inst, tile_ado0, tile_nado, step_ctr, nb_up, nb_dn, pidx;
     DevBuf<uint8_t> vreal;
-    DevBuf<double2> V, H, Xp, field, R, P, wr, wc, damp, Y, ACC, S1, S2;
+    DevBuf<double> cx, cy;
+    DevBuf<double2> V, H, Xp, field, R, P, alpha, alpha_t, damp, Y, ACC, S1, S2;
     DevBuf<long long> ado0_off;
     std::vector<Group> groups;
     int nchunks = 0;
@@ -511,6 +614,44 @@ static int launch_stage(const Group& g, const StageArgs& base, int tile0, hipStr
     return EH_OK;
 }
 
+static StageArgs stage_args(const BatchImpl* im) {
+    StageArgs sa;
+    std::memset(&sa, 0, sizeof(sa));
+    sa.inst = im->inst.p;
+    sa.tile_inst = im->tile_inst.p;
+    sa.tile_ado0 = im->tile_ado0.p;
+    sa.tile_nado = im->tile_nado.p;
+    sa.step_ctr = im->step_ctr.p;
+    sa.vreal = im->vreal.p;
+    sa.V = im->V.p;
+    sa.nb_up = im->nb_up.p;
+    sa.nb_dn = im->nb_dn.p;
+    sa.cx = im->cx.p;
+    sa.cy = im->cy.p;
+    sa.alpha = im->alpha.p;
+    sa.alpha_t = im->alpha_t.p;
+    sa.damp = im->damp.p;
+    sa.y = im->Y.p;
+    sa.acc = im->ACC.p;
+    return sa;
+}
+
+static PrologueArgs prologue_args(const BatchImpl* im) {
+    PrologueArgs pa;
+    std::memset(&pa, 0, sizeof(pa));
+    pa.inst = im->inst.p;
+    pa.step_ctr = im->step_ctr.p;
+    pa.H = im->H.p;
+    pa.Xp = im->Xp.p;
+    pa.field = im->field.p;
+    pa.pidx = im->pidx.p;
+    pa.R = im->R.p;
+    pa.y = im->Y.p;
+    pa.V = im->V.p;
+    pa.P = im->P.p;
+    return pa;
+}
+
 int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
     if (!ctx || !in || n <= 0 || !out) return fail(EH_EINVAL, "batch_create: bad arguments");
     HIPCHK(hipSetDevice(ctx->device));
@@ -522,13 +663,13 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
 
     std::vector<InstDev> hinst(n);
     std::vector<int> t_inst, t_ado0, t_nado;
-    std::vector<int> lptr, lnbr, lsite, hpidx;
+    std::vector<int> hup, hdn, hpidx;
+    std::vector<double> hcx, hcy;
     std::vector<uint8_t> hvreal;
-    std::vector<double2> hH, hXp, hfield, hR, hwr, hwc, hdamp;
+    std::vector<double2> hH, hXp, hfield, hR, hal, halt, hdamp;
     std::vector<long long> hado0;
-    lptr.push_back(0);
-    int64_t state_off = 0, p_off = 0;
-    int ado_off = 0, v_off = 0, r_off = 0, step_off = 0;
+    int64_t state_off = 0, p_off = 0, lk_off = 0;
+    int ado_off = 0, v_off = 0, r_off = 0, step_off = 0, al_off = 0;
     // tiles grouped by kernel variant
     // kernel variants: 1..16 stage_small<N>, 17 = stage_mfma<32>, 18 = stage_mfma<64>, 0 = stage_generic
     std::vector<std::vector<int>> tiles_by_nt(19);   // each entry: triples (inst, ado0, nado)
@@ -547,8 +688,8 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
         Hierarchy h;
         int rc = build_hierarchy(sys.K * sys.Km, sys.L, h);
         if (rc) return rc;
-        Couplings c;
-        rc = build_couplings(h, sys, c);
+        Tables tb;
+        rc = build_tables(h, sys, tb);
         if (rc) return rc;
         Schedule s;
         rc = build_schedule(sys, q.mode, q.nv, s);
@@ -589,16 +730,14 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
             hpidx.push_back(s.pidx[(size_t)st]);
             hvreal.push_back((real_h && !s.active[(size_t)st]) ? 1 : 0);
         }
-        for (int m = 0; m < M; m++) {
-            hdamp.push_back(d2(c.damp[(size_t)m]));
-            for (int l = c.ptr[(size_t)m]; l < c.ptr[(size_t)m + 1]; l++) {
-                const Link& lk = c.links[(size_t)l];
-                lnbr.push_back(lk.nbr);
-                lsite.push_back(lk.site);
-                hwr.push_back(d2(lk.w_row));
-                hwc.push_back(d2(lk.w_col));
-            }
-            lptr.push_back((int)lnbr.size());
+        for (int m = 0; m < M; m++) hdamp.push_back(d2(tb.damp[(size_t)m]));
+        hup.insert(hup.end(), tb.up.begin(), tb.up.end());
+        hdn.insert(hdn.end(), tb.dn.begin(), tb.dn.end());
+        hcx.insert(hcx.end(), tb.cx.begin(), tb.cx.end());
+        hcy.insert(hcy.end(), tb.cy.begin(), tb.cy.end());
+        for (int e = 0; e < sys.K * sys.Km; e++) {
+            hal.push_back(make_double2(sys.alpha[2 * e], sys.alpha[2 * e + 1]));
+            halt.push_back(make_double2(sys.alpha_t[2 * e], sys.alpha_t[2 * e + 1]));
         }
         InstDev& d = hinst[i];
         d.state_off = state_off;
@@ -611,8 +750,13 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
         d.step_off = step_off;
         d.nread = nread;
         d.npoints = s.npoints;
-        d.real_h = real_h ? 1 : 0;
         d.r_off = r_off;
+        d.lk_off = lk_off;
+        d.d = h.d;
+        d.K = sys.K;
+        d.Km = sys.Km;
+        d.so = sys.site_offset;
+        d.al_off = al_off;
         d.h2 = s.h2;
         d.h = s.h;
         for (int k = 0; k < 4; k++) d.hb[k] = s.hb[k];
@@ -643,6 +787,8 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
         v_off += NN;
         r_off += nread * NN;
         step_off += s.nsteps;
+        lk_off += (int64_t)M * h.d;
+        al_off += sys.K * sys.Km;
     }
     b->total_ados = ado_off;
     b->total_elems = state_off;
@@ -653,19 +799,22 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
         g.nt = nt <= 16 ? nt : (nt == 17 ? -32 : -64);
         g.tile_begin = (int)t_inst.size();
         g.tile_count = (int)(tv.size() / 3);
-        int maxnn = 0;
+        int maxnn = 0, maxkn = 0;
         for (size_t k = 0; k < tv.size(); k += 3) {
             t_inst.push_back(tv[k]);
             t_ado0.push_back(tv[k + 1]);
             t_nado.push_back(tv[k + 2]);
-            maxnn = std::max(maxnn, hinst[(size_t)tv[k]].N * hinst[(size_t)tv[k]].N);
+            const InstDev& di = hinst[(size_t)tv[k]];
+            maxnn = std::max(maxnn, di.N * di.N);
+            maxkn = std::max(maxkn, 2 * di.K * di.N);
         }
+        const size_t cpb = (size_t)maxkn * sizeof(double2);   // coupling rows/columns
         if (g.nt == -32)
-            g.shmem = (size_t)2 * 32 * 33 * sizeof(double2);
+            g.shmem = (size_t)2 * 32 * 33 * sizeof(double2) + cpb;
         else if (g.nt == -64)
-            g.shmem = (size_t)64 * 65 * sizeof(double2);
+            g.shmem = (size_t)64 * 65 * sizeof(double2) + cpb;
         else
-            g.shmem = nt ? 0 : (size_t)maxnn * sizeof(double2);
+            g.shmem = nt ? 0 : (size_t)maxnn * sizeof(double2) + cpb;
         if ((int)g.shmem > ctx->lds_per_block) return fail(EH_ESHAPE, "state tile exceeds LDS");
         if (g.shmem > 65536) {
             hipError_t fe = g.nt == -64 ? hipFuncSetAttribute((const void*)stage_mfma<64, false>,
@@ -680,11 +829,11 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
     int rc;
     std::vector<InstDev> hi = hinst;
     if ((rc = im->inst.upload(hi)) || (rc = im->tile_inst.upload(t_inst)) || (rc = im->tile_ado0.upload(t_ado0)) ||
-        (rc = im->tile_nado.upload(t_nado)) || (rc = im->link_ptr.upload(lptr)) || (rc = im->link_nbr.upload(lnbr)) ||
-        (rc = im->link_site.upload(lsite)) || (rc = im->pidx.upload(hpidx)) || (rc = im->vreal.upload(hvreal)) ||
-        (rc = im->H.upload(hH)) || (rc = im->Xp.upload(hXp)) || (rc = im->field.upload(hfield)) ||
-        (rc = im->R.upload(hR)) || (rc = im->wr.upload(hwr)) || (rc = im->wc.upload(hwc)) ||
-        (rc = im->damp.upload(hdamp)) || (rc = im->ado0_off.upload(hado0)))
+        (rc = im->tile_nado.upload(t_nado)) || (rc = im->nb_up.upload(hup)) || (rc = im->nb_dn.upload(hdn)) ||
+        (rc = im->cx.upload(hcx)) || (rc = im->cy.upload(hcy)) || (rc = im->pidx.upload(hpidx)) ||
+        (rc = im->vreal.upload(hvreal)) || (rc = im->H.upload(hH)) || (rc = im->Xp.upload(hXp)) ||
+        (rc = im->field.upload(hfield)) || (rc = im->R.upload(hR)) || (rc = im->alpha.upload(hal)) ||
+        (rc = im->alpha_t.upload(halt)) || (rc = im->damp.upload(hdamp)) || (rc = im->ado0_off.upload(hado0)))
         return rc;
     if ((rc = im->step_ctr.alloc(1)) || (rc = im->V.alloc(hH.size())) || (rc = im->P.alloc((size_t)p_off)) ||
         (rc = im->Y.alloc((size_t)state_off)) || (rc = im->ACC.alloc((size_t)state_off)) ||
@@ -698,33 +847,8 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
     const int per_step = 1 + 4 * (int)im->groups.size();
     b->chunk = std::max(1, std::min(b->max_steps, 256 / per_step));
     im->nchunks = (b->max_steps + b->chunk - 1) / b->chunk;
-    StageArgs sa;
-    sa.inst = im->inst.p;
-    sa.tile_inst = im->tile_inst.p;
-    sa.tile_ado0 = im->tile_ado0.p;
-    sa.tile_nado = im->tile_nado.p;
-    sa.step_ctr = im->step_ctr.p;
-    sa.vreal = im->vreal.p;
-    sa.V = im->V.p;
-    sa.link_ptr = im->link_ptr.p;
-    sa.link_nbr = im->link_nbr.p;
-    sa.link_site = im->link_site.p;
-    sa.link_wr = im->wr.p;
-    sa.link_wc = im->wc.p;
-    sa.damp = im->damp.p;
-    sa.y = im->Y.p;
-    sa.acc = im->ACC.p;
-    PrologueArgs pa;
-    pa.inst = im->inst.p;
-    pa.step_ctr = im->step_ctr.p;
-    pa.H = im->H.p;
-    pa.Xp = im->Xp.p;
-    pa.field = im->field.p;
-    pa.pidx = im->pidx.p;
-    pa.R = im->R.p;
-    pa.y = im->Y.p;
-    pa.V = im->V.p;
-    pa.P = im->P.p;
+    const StageArgs sa = stage_args(im.get());
+    PrologueArgs pa = prologue_args(im.get());
     HIPCHK(hipStreamBeginCapture(im->stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < b->chunk; k++) {
         pa.step_off = k;
@@ -817,34 +941,8 @@ int batch_profile(Batch* b, int nsteps, double* out) {
                        im->ado0_off.p, b->n_inst);
     std::vector<hipEvent_t> ev((size_t)6 * nsteps);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-    StageArgs sa;
-    std::memset(&sa, 0, sizeof(sa));
-    sa.inst = im->inst.p;
-    sa.tile_inst = im->tile_inst.p;
-    sa.tile_ado0 = im->tile_ado0.p;
-    sa.tile_nado = im->tile_nado.p;
-    sa.step_ctr = im->step_ctr.p;
-    sa.vreal = im->vreal.p;
-    sa.V = im->V.p;
-    sa.link_ptr = im->link_ptr.p;
-    sa.link_nbr = im->link_nbr.p;
-    sa.link_site = im->link_site.p;
-    sa.link_wr = im->wr.p;
-    sa.link_wc = im->wc.p;
-    sa.damp = im->damp.p;
-    sa.y = im->Y.p;
-    sa.acc = im->ACC.p;
-    PrologueArgs pa;
-    pa.inst = im->inst.p;
-    pa.step_ctr = im->step_ctr.p;
-    pa.H = im->H.p;
-    pa.Xp = im->Xp.p;
-    pa.field = im->field.p;
-    pa.pidx = im->pidx.p;
-    pa.R = im->R.p;
-    pa.y = im->Y.p;
-    pa.V = im->V.p;
-    pa.P = im->P.p;
+    const StageArgs sa = stage_args(im);
+    PrologueArgs pa = prologue_args(im);
     pa.step_off = 0;
     int rc = EH_OK;
     for (int k = 0; k < nsteps && rc == EH_OK; k++) {

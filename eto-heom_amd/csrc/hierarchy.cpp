@@ -1,10 +1,10 @@
-// hierarchy.cpp -- ADO enumeration, the CSR coupling table and the per-step schedule.
+// hierarchy.cpp -- ADO enumeration, the neighbour/coefficient tables and the per-step schedule.
 //
 // The reference enumerates the multi-indices by DFS (etoheom-cpu/lib/utilize.cpp:3-18),
 // which is the lexicographic order of {n in N^d : |n| <= L}; it then finds n +/- e_p
 // by string edits and hash lookups on every RHS evaluation (dynamics.cpp:143-179).
-// Here the lookups happen once, into a CSR table of links whose complex weights
-// already carry the reference's scaled coefficients, so the kernels only gather.
+// Here the lookups happen once, into (ADO, p) tables of neighbour indices and the
+// reference's scaled coefficients, so the kernels only gather.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -68,17 +68,19 @@ int64_t Hierarchy::index_of(const uint8_t* key) const {
     return -1;
 }
 
-int build_couplings(const Hierarchy& h, const eh_system& sys, Couplings& c) {
+int build_tables(const Hierarchy& h, const eh_system& sys, Tables& t) {
     const int K = sys.K, Km = sys.Km, d = h.d;
-    const cplx I(0, 1);
     auto get = [](const double* a, int i) { return cplx(a[2 * i], a[2 * i + 1]); };
     for (int j = 0; j < K; j++)
         for (int k = 0; k < Km; k++)
             if (std::abs(get(sys.alpha, j * Km + k)) == 0.0)
                 return fail(EH_EINVAL, "an ETO mode has |alpha| = 0 (the reference divides by it)");
-    c.ptr.assign((size_t)h.M + 1, 0);
-    c.links.clear();
-    c.damp.assign((size_t)h.M, cplx(0, 0));
+    const size_t md = (size_t)h.M * d;
+    t.up.assign(md, -1);
+    t.dn.assign(md, -1);
+    t.cx.assign(md, 0.0);
+    t.cy.assign(md, 0.0);
+    t.damp.assign((size_t)h.M, cplx(0, 0));
     std::vector<uint8_t> tmp((size_t)(d ? d : 1));
     for (int64_t i = 0; i < h.M; i++) {
         const uint8_t* n = h.ado.data() + i * d;
@@ -93,35 +95,30 @@ int build_couplings(const Hierarchy& h, const eh_system& sys, Couplings& c) {
                     if (n[off] == 0) continue;
                     x += cplx(-1, 0) * (get(sys.gamma, j * Km + k) * (double)n[off]);
                 }
-            c.damp[(size_t)i] = x;
+            t.damp[(size_t)i] = x;
         }
-        // up: x ( i S rho - i rho S ),  x = sqrt((n_p + 1) |alpha_p|)   (dynamics.cpp:143-156)
         for (int j = 0; j < K; j++)
             for (int k = 0; k < Km; k++) {
-                const int off = j + k * K;
-                if (n[off] >= sys.L || sum >= sys.L) continue;
-                std::memcpy(tmp.data(), n, (size_t)d);
-                tmp[off] += 1;
-                const int64_t m = h.index_of(tmp.data());
-                if (m < 0) return fail(EH_ESHAPE, "missing up neighbour");
-                const double x = std::sqrt(((double)n[off] + 1) * std::abs(get(sys.alpha, j * Km + k)));
-                c.links.push_back({(int32_t)m, j + sys.site_offset, I * x, -I * x});
-            }
-        // down: y ( i alpha S rho - i alpha~ rho S ),  y = sqrt(n_p / |alpha_p|)   (dynamics.cpp:160-179)
-        if (i != 0)
-            for (int j = 0; j < K; j++)
-                for (int k = 0; k < Km; k++) {
-                    const int off = j + k * K;
-                    if (n[off] == 0) continue;
+                const int p = j + k * K;
+                const size_t q = (size_t)i * d + p;
+                const double aabs = std::abs(get(sys.alpha, j * Km + k));
+                if (n[p] < sys.L && sum < sys.L) {   // n + e_p exists (dynamics.cpp:146-149)
                     std::memcpy(tmp.data(), n, (size_t)d);
-                    tmp[off] -= 1;
+                    tmp[p] += 1;
+                    const int64_t m = h.index_of(tmp.data());
+                    if (m < 0) return fail(EH_ESHAPE, "missing up neighbour");
+                    t.up[q] = (int32_t)m;
+                    t.cx[q] = std::sqrt(((double)n[p] + 1) * aabs);
+                }
+                if (i != 0 && n[p] > 0) {             // n - e_p (dynamics.cpp:164-168)
+                    std::memcpy(tmp.data(), n, (size_t)d);
+                    tmp[p] -= 1;
                     const int64_t m = h.index_of(tmp.data());
                     if (m < 0) return fail(EH_ESHAPE, "missing down neighbour");
-                    const cplx a = get(sys.alpha, j * Km + k), at = get(sys.alpha_t, j * Km + k);
-                    const double y = std::sqrt((double)n[off] / std::abs(a));
-                    c.links.push_back({(int32_t)m, j + sys.site_offset, I * (a * y), -I * (at * y)});
+                    t.dn[q] = (int32_t)m;
+                    t.cy[q] = std::sqrt((double)n[p] / aabs);
                 }
-        c.ptr[(size_t)i + 1] = (int32_t)c.links.size();
+            }
     }
     return EH_OK;
 }

@@ -63,18 +63,17 @@ struct Hierarchy {
 int64_t hierarchy_size(int d, int L);
 int build_hierarchy(int d, int L, Hierarchy& h);
 
-// One coupling of ADO n to a neighbour m:  drho_n[a][b] += ([a==site] w_row + [b==site] w_col) rho_m[a][b]
-struct Link {
-    int32_t nbr;
-    int32_t site;
-    cplx w_row, w_col;
+// Coupling tables in (ADO, p) layout, p = j + k K (bath j, mode k):
+//   up[n][p] = index of n + e_p or -1,  cx[n][p] = sqrt((n_p + 1) |alpha_p|)
+//   dn[n][p] = index of n - e_p or -1,  cy[n][p] = sqrt(n_p / |alpha_p|)
+// and damp[n] = -(sum_p n_p gamma_p) (zero for n = 0), which together carry the
+// reference's scaled coefficients (dynamics.cpp:129-179).
+struct Tables {
+    std::vector<int32_t> up, dn;
+    std::vector<double> cx, cy;
+    std::vector<cplx> damp;
 };
-struct Couplings {
-    std::vector<int32_t> ptr;   // M+1
-    std::vector<Link> links;
-    std::vector<cplx> damp;     // -(sum_p n_p gamma_p), zero for n = 0
-};
-int build_couplings(const Hierarchy& h, const eh_system& sys, Couplings& c);
+int build_tables(const Hierarchy& h, const eh_system& sys, Tables& t);
 
 // Host-side per-step schedule of one propagation: float time grid, field scalars,
 // readout slots (dynamics.cpp:344-418, TD_hamiltonian.cpp:5-31).
