@@ -121,6 +121,7 @@ def main():
     wall = time.perf_counter() - t0
     gpu_ms = batch.elapsed_ms()
     outs = [batch.output(i) for i in range(len(items))]
+    rho_final = batch.rho(0) if not a.no_cpu else None   # dense end-of-run state for the CPU sample
     finite = all(np.all(np.isfinite(o)) for o in outs)
 
     # per-launch durations of the stage kernel (HIP events on the batch stream)
@@ -156,7 +157,7 @@ def main():
                      "prologue_ms": float(prof[0]), "bytes_per_launch": bytes_per_launch},
     }
     if not a.no_cpu:
-        rec["cpu_baseline"] = cpu_baseline(items[0], batch.rho(0))
+        rec["cpu_baseline"] = cpu_baseline(items[0], rho_final)
     print(json.dumps(rec))
     return 0
 
