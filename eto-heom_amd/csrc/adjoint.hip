@@ -1,0 +1,735 @@
+// adjoint.hip -- discrete adjoint of the batched RK4 HEOM propagation.
+//
+// Forward step i (V_i frozen, A_i the hierarchy generator of the RHS):
+//   Y1 = y, k1 = A Y1, Y2 = y + h/2 k1, k2 = A Y2, Y3 = y + h/2 k2, k3 = A Y3,
+//   Y4 = y + h k3, k4 = A Y4, y' = y + h/6 k1 + h/3 k2 + h/3 k3 + h/6 k4,
+// with readouts P_i = Tr(rho_0(y_i) X3) before the step.  For a real loss L and
+// the gradient convention  g = dL/dRe z + i dL/dIm z,  the exact reverse step is
+//   kb4 = h/6 l'           U4 = A^H kb4     (stage A: input l')
+//   kb3 = h/3 l' + h U4    U3 = A^H kb3     (stage B)
+//   kb2 = h/3 l' + h/2 U3  U2 = A^H kb2     (stage C)
+//   kb1 = h/6 l' + h/2 U2  U1 = A^H kb1     (stage D)
+//   l = l' + U1 + U2 + U3 + U4,  then  l[ADO 0][j][k] += conj(X3[k][j]) g_i.
+// A^H is the transposed hierarchy: i (V^H l - l V^H), conj(damping), and the
+// coupling of ADO m gathers l at m - e_p with the conjugated up-weights of that
+// ADO and l at m + e_p with the conjugated down-weights (so the up and down
+// tables swap roles; see build of the adjoint tables in adj_enable).
+//
+// Parameter gradients accumulate sum_i sum_s Re <kb_s, (dA/dtheta) Y_s> on the
+// device, per instance:
+//   Gt  = sum (Y_n kb_n^H - kb_n^H Y_n)             -> dL/dV = (-i Gt)^H
+//   Gq  = sum e_q(t_i) (...)                        -> dL/dX'_q
+//   Sg_p = sum_n n_p <kb_n, Y_n>                    -> dL/dgamma_p = -conj(Sg_p)
+//   U_p = sum x R_up, A_p = sum y R_dnrow, B_p = sum y R_dncol  -> dL/dalpha, dL/dalpha~
+// and the readout term  dL/dX3[k][j] += conj(rho_0[j][k]) g_i.
+//
+// Memory: the forward run stores y every C steps; the backward replays each
+// segment forward from its checkpoint, keeping every stage input of the
+// segment, then walks the segment backwards.  Both are graph-captured.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "engine_impl.hpp"
+
+namespace eh {
+
+struct AdjArgs {
+    StageArgs S;             // adjoint coupling tables, conj(alpha), conj(damp), V = V^H, gsign = -1
+    const double2* Yf;       // forward stage input paired with this adjoint stage
+    const int* fnb_up;       // forward tables (for the coupling-parameter reductions)
+    const int* fnb_dn;
+    const double* fcx;
+    const double* fcy;
+    const uint8_t* ado;      // multi-indices in (ADO, p) layout
+    const double2* field;    // per step e_q
+    double2* gacc;           // per instance accumulators
+    const long long* gacc_off;
+};
+
+// layout of one instance's accumulator block (complex entries)
+__host__ __device__ inline long long gacc_size(int NN, int d) { return 5LL * NN + 4LL * d; }
+// [0, NN) Gt | [NN, 4NN) G_q weighted by e_q | [4NN, 5NN) GX3 | Sg[d] | U[d] | A[d] | B[d]
+
+__device__ __forceinline__ void atomic_add2(double2* p, double2 v) {
+    atomicAdd(&p->x, v.x);
+    atomicAdd(&p->y, v.y);
+}
+
+__device__ __forceinline__ double2 cconj_mul(double2 a, double2 b) {   // conj(a) * b
+    return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+
+// adjoint RK stage update of one element (coefficients of the reverse step above)
+__device__ __forceinline__ void rk_store_adj(const StageArgs& A, const InstDev& I, size_t gi, double2 lam, double2 acc,
+                                             double ur, double ui) {
+    const int st = A.stage;
+    const double h = I.h;
+    if (st == 0) {
+        const double a = h / 6.0, b = h / 3.0, c = h * h / 6.0;
+        A.acc[gi] = make_double2(fma(a, ur, lam.x), fma(a, ui, lam.y));
+        A.out[gi] = make_double2(fma(c, ur, b * lam.x), fma(c, ui, b * lam.y));
+    } else if (st < 3) {
+        const double b = st == 1 ? h / 3.0 : h / 6.0, c = h / 2.0;
+        A.acc[gi] = make_double2(acc.x + ur, acc.y + ui);
+        A.out[gi] = make_double2(fma(c, ur, b * lam.x), fma(c, ui, b * lam.y));
+    } else {
+        A.out[gi] = make_double2(acc.x + ur, acc.y + ui);
+    }
+}
+
+// Reductions of the coupling-parameter terms for the ADOs [n0, n0+na) of one
+// instance: entries (n, j, rc, pos) over row / column s = j + so, pairing kb_n
+// (sKb, scaled by fs) with the forward stage input of the neighbours.
+__device__ void reduce_bath(const AdjArgs& J, const InstDev& I, int n0, int na, const double2* sKb, int kstride,
+                            double fs, double* sU, int tid, int nthr) {
+    const int N = I.N, NN = N * N, K = I.K, Km = I.Km, d = I.d;
+    const int per = 2 * K * N;
+    for (int e = tid; e < per * na; e += nthr) {
+        const int g = e / per, eg = e - g * per;
+        const int j = eg / (2 * N), rem = eg - j * 2 * N;
+        const int rc = rem / N, pos = rem - rc * N;
+        const int s = j + I.so;
+        const int ab = rc == 0 ? s * N + pos : pos * N + s;
+        double2 kb = sKb[g * kstride + ab];
+        kb.x *= fs;
+        kb.y *= fs;
+        const long long row0 = I.lk_off + (long long)(n0 + g) * d;
+        for (int k = 0; k < Km; ++k) {
+            const int p = j + k * K;
+            const long long q = row0 + p;
+            const int mu = J.fnb_up[q], md = J.fnb_dn[q];
+            if (mu >= 0) {
+                const double2 yu = J.Yf[I.state_off + (long long)mu * NN + ab];
+                double2 t = cconj_mul(kb, yu);
+                const double x = J.fcx[q] * (rc == 0 ? 1.0 : -1.0);
+                // x * (+-i) * conj(kb) * yu
+                atomicAdd(&sU[2 * p], -x * t.y);
+                atomicAdd(&sU[2 * p + 1], x * t.x);
+            }
+            if (md >= 0) {
+                const double2 yd = J.Yf[I.state_off + (long long)md * NN + ab];
+                double2 t = cconj_mul(kb, yd);
+                const double y = J.fcy[q] * (rc == 0 ? 1.0 : -1.0);
+                double* dst = sU + 2 * d * (rc == 0 ? 1 : 2);   // A_p (row) or B_p (column)
+                atomicAdd(&dst[2 * p], -y * t.y);
+                atomicAdd(&dst[2 * p + 1], y * t.x);
+            }
+        }
+    }
+}
+
+template <int NT>
+__global__ __launch_bounds__(256) void adj_small(AdjArgs J) {
+    const StageArgs& A = J.S;
+    constexpr int NN = NT * NT;
+    constexpr int G = NN >= 512 ? 1 : 512 / NN;
+    constexpr int EPT = (G * NN + 255) / 256;
+    constexpr int CPMAX = G * 2 * NT * NT;
+    constexpr int DMAX = 64;
+    __shared__ double2 sR[G * NN];
+    __shared__ double2 sY[G * NN];
+    __shared__ double2 sV[NN];
+    __shared__ double2 sC[CPMAX];
+    __shared__ double2 sG[G * NN];
+    __shared__ double2 sQ[G * NN];
+    __shared__ double sU[3 * 2 * DMAX];
+    const int tile = blockIdx.x;
+    const int ii = A.tile_inst[tile];
+    const InstDev& I = A.inst[ii];
+    const int step = *A.step_ctr + A.step_off;
+    if (step < 0 || step >= I.nsteps) return;
+    const int a0 = A.tile_ado0[tile], na = A.tile_nado[tile];
+    const long long base = I.state_off + (long long)a0 * NN;
+    const int nel = na * NN;
+    const int st = A.stage;
+    double2 rv[EPT], lv[EPT], av[EPT];
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+        const int e = threadIdx.x + q * 256;
+        rv[q] = make_double2(0.0, 0.0);
+        lv[q] = make_double2(0.0, 0.0);
+        av[q] = make_double2(0.0, 0.0);
+        if (e < nel) {
+            rv[q] = A.rin[base + e];
+            if (st == 1 || st == 2) lv[q] = A.y[base + e];
+            if (st > 0) av[q] = A.acc[base + e];
+            sY[e] = J.Yf[base + e];
+        }
+    }
+    for (int e = threadIdx.x; e < NN; e += 256) sV[e] = A.V[I.v_off + e];
+    for (int e = threadIdx.x; e < 6 * I.d; e += 256) sU[e] = 0.0;
+    gather_couplings(A, I, a0, na, sC, threadIdx.x, 256);
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+        const int e = threadIdx.x + q * 256;
+        if (e < nel) sR[e] = rv[q];
+    }
+    __syncthreads();
+    const bool vreal = A.vreal[I.step_off + step] != 0;
+    const double fs = st == 0 ? I.h / 6.0 : 1.0;   // kb = fs * stage input
+    const int cpn = 2 * I.K * NT;
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+        const int e = threadIdx.x + q * 256;
+        if (e >= nel) break;
+        const int nl = e / NN, ab = e - nl * NN, ra = ab / NT, cb = ab - ra * NT;
+        const double2* R = sR + nl * NN;
+        const double2* Y = sY + nl * NN;
+        double cr = 0.0, ci = 0.0;   // (V^H l - l V^H)[ra][cb]
+        double2 gt = make_double2(0.0, 0.0);
+        if (vreal) {
+#pragma unroll
+            for (int c = 0; c < NT; ++c) {
+                const double vac = sV[ra * NT + c].x, vcb = sV[c * NT + cb].x;
+                const double2 rcb = R[c * NT + cb], rac = R[ra * NT + c];
+                cr = fma(vac, rcb.x, fma(-rac.x, vcb, cr));
+                ci = fma(vac, rcb.y, fma(-rac.y, vcb, ci));
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < NT; ++c) {
+                const double2 vac = sV[ra * NT + c], vcb = sV[c * NT + cb];
+                const double2 rcb = R[c * NT + cb], rac = R[ra * NT + c];
+                cr = fma(vac.x, rcb.x, fma(-vac.y, rcb.y, cr));
+                ci = fma(vac.x, rcb.y, fma(vac.y, rcb.x, ci));
+                cr = fma(-rac.x, vcb.x, fma(rac.y, vcb.y, cr));
+                ci = fma(-rac.x, vcb.y, fma(-rac.y, vcb.x, ci));
+            }
+        }
+        // Gt[ra][cb] = sum_c Y[ra][c] conj(kb[cb][c]) - conj(kb[c][ra]) Y[c][cb]
+#pragma unroll
+        for (int c = 0; c < NT; ++c) {
+            const double2 y1 = Y[ra * NT + c], k1 = R[cb * NT + c];
+            const double2 k2 = R[c * NT + ra], y2 = Y[c * NT + cb];
+            gt.x += y1.x * k1.x + y1.y * k1.y - (k2.x * y2.x + k2.y * y2.y);
+            gt.y += y1.y * k1.x - y1.x * k1.y - (k2.x * y2.y - k2.y * y2.x);
+        }
+        sG[e] = make_double2(fs * gt.x, fs * gt.y);
+        const double2 qv = cconj_mul(R[ab], Y[ab]);
+        sQ[e] = make_double2(fs * qv.x, fs * qv.y);
+        double ur = -ci, ui = cr;   // +i * C'
+        const double2 g = A.damp[I.ado_off + a0 + nl], r = R[ab];
+        ur = fma(g.x, r.x, fma(-g.y, r.y, ur));
+        ui = fma(g.x, r.y, fma(g.y, r.x, ui));
+        const double2 cpl = coupling_of(I, sC + nl * cpn, ra, cb);
+        const double2 lam = st == 0 ? r : lv[q];
+        rk_store_adj(A, I, (size_t)(base + e), lam, av[q], ur + cpl.x, ui + cpl.y);
+    }
+    reduce_bath(J, I, a0, na, sR, NN, fs, sU, threadIdx.x, 256);
+    __syncthreads();
+    double2* acc = J.gacc + J.gacc_off[ii];
+    const double2* ef = J.field + 3 * (size_t)(I.step_off + step);
+    const double2 e0 = ef[0], e1 = ef[1], e2 = ef[2];
+    for (int ab = threadIdx.x; ab < NN; ab += 256) {
+        double2 gs = make_double2(0.0, 0.0);
+        for (int n = 0; n < na; ++n) {
+            gs.x += sG[n * NN + ab].x;
+            gs.y += sG[n * NN + ab].y;
+        }
+        atomic_add2(acc + ab, gs);
+        atomic_add2(acc + NN + ab, cmad(make_double2(0.0, 0.0), e0, gs));
+        atomic_add2(acc + 2 * NN + ab, cmad(make_double2(0.0, 0.0), e1, gs));
+        atomic_add2(acc + 3 * NN + ab, cmad(make_double2(0.0, 0.0), e2, gs));
+    }
+    const int d = I.d;
+    for (int t = threadIdx.x; t < na * d; t += 256) {
+        const int n = t / d, p = t - n * d;
+        const int np_ = J.ado[I.lk_off + (long long)(a0 + n) * d + p];
+        if (np_ == 0) continue;
+        double2 qn = make_double2(0.0, 0.0);
+        for (int ab = 0; ab < NN; ++ab) {
+            qn.x += sQ[n * NN + ab].x;
+            qn.y += sQ[n * NN + ab].y;
+        }
+        atomic_add2(acc + 5 * NN + p, make_double2(np_ * qn.x, np_ * qn.y));
+    }
+    for (int t = threadIdx.x; t < 3 * d; t += 256) {
+        const int w = t / d, p = t - w * d;
+        atomic_add2(acc + 5 * NN + (1 + w) * d + p, make_double2(sU[2 * (w * d + p)], sU[2 * (w * d + p) + 1]));
+    }
+}
+
+// any N <= 64: one ADO per workgroup (VALU); dynamic LDS: l-input, Y, couplings
+__global__ __launch_bounds__(256) void adj_generic(AdjArgs J) {
+    const StageArgs& A = J.S;
+    extern __shared__ double2 smem[];
+    const int tile = blockIdx.x;
+    const int ii = A.tile_inst[tile];
+    const InstDev& I = A.inst[ii];
+    const int step = *A.step_ctr + A.step_off;
+    if (step < 0 || step >= I.nsteps) return;
+    const int N = I.N, NN = N * N, d = I.d;
+    double2* sR = smem;
+    double2* sY = smem + NN;
+    double2* sC = smem + 2 * NN;
+    double* sU = reinterpret_cast<double*>(smem + 2 * NN + 2 * I.K * N);
+    __shared__ double2 red[256];
+    const int a0 = A.tile_ado0[tile];
+    const int st = A.stage;
+    const long long base = I.state_off + (long long)a0 * NN;
+    for (int e = threadIdx.x; e < NN; e += blockDim.x) {
+        sR[e] = A.rin[base + e];
+        sY[e] = J.Yf[base + e];
+    }
+    for (int e = threadIdx.x; e < 6 * d; e += blockDim.x) sU[e] = 0.0;
+    gather_couplings(A, I, a0, 1, sC, threadIdx.x, blockDim.x);
+    __syncthreads();
+    const double2* V = A.V + I.v_off;
+    const bool vreal = A.vreal[I.step_off + step] != 0;
+    const double fs = st == 0 ? I.h / 6.0 : 1.0;
+    const double2 g = A.damp[I.ado_off + a0];
+    double2* acc = J.gacc + J.gacc_off[ii];
+    const double2* ef = J.field + 3 * (size_t)(I.step_off + step);
+    const double2 e0 = ef[0], e1 = ef[1], e2 = ef[2];
+    double2 qsum = make_double2(0.0, 0.0);
+    for (int ab = threadIdx.x; ab < NN; ab += blockDim.x) {
+        const int ra = ab / N, cb = ab - ra * N;
+        double cr = 0.0, ci = 0.0;
+        double2 gt = make_double2(0.0, 0.0);
+        for (int c = 0; c < N; ++c) {
+            const double2 vac = V[ra * N + c], vcb = V[c * N + cb];
+            const double2 rcb = sR[c * N + cb], rac = sR[ra * N + c];
+            if (vreal) {
+                cr = fma(vac.x, rcb.x, fma(-rac.x, vcb.x, cr));
+                ci = fma(vac.x, rcb.y, fma(-rac.y, vcb.x, ci));
+            } else {
+                cr = fma(vac.x, rcb.x, fma(-vac.y, rcb.y, cr));
+                ci = fma(vac.x, rcb.y, fma(vac.y, rcb.x, ci));
+                cr = fma(-rac.x, vcb.x, fma(rac.y, vcb.y, cr));
+                ci = fma(-rac.x, vcb.y, fma(-rac.y, vcb.x, ci));
+            }
+            const double2 y1 = sY[ra * N + c], k1 = sR[cb * N + c];
+            const double2 k2 = sR[c * N + ra], y2 = sY[c * N + cb];
+            gt.x += y1.x * k1.x + y1.y * k1.y - (k2.x * y2.x + k2.y * y2.y);
+            gt.y += y1.y * k1.x - y1.x * k1.y - (k2.x * y2.y - k2.y * y2.x);
+        }
+        gt = make_double2(fs * gt.x, fs * gt.y);
+        atomic_add2(acc + ab, gt);
+        atomic_add2(acc + NN + ab, cmad(make_double2(0.0, 0.0), e0, gt));
+        atomic_add2(acc + 2 * NN + ab, cmad(make_double2(0.0, 0.0), e1, gt));
+        atomic_add2(acc + 3 * NN + ab, cmad(make_double2(0.0, 0.0), e2, gt));
+        const double2 qv = cconj_mul(sR[ab], sY[ab]);
+        qsum.x += fs * qv.x;
+        qsum.y += fs * qv.y;
+        double ur = -ci, ui = cr;
+        const double2 r = sR[ab];
+        ur = fma(g.x, r.x, fma(-g.y, r.y, ur));
+        ui = fma(g.x, r.y, fma(g.y, r.x, ui));
+        const double2 cpl = coupling_of(I, sC, ra, cb);
+        const double2 lam = st == 0 ? r : (st < 3 ? A.y[base + ab] : make_double2(0.0, 0.0));
+        const double2 av = st > 0 ? A.acc[base + ab] : make_double2(0.0, 0.0);
+        rk_store_adj(A, I, (size_t)(base + ab), lam, av, ur + cpl.x, ui + cpl.y);
+    }
+    red[threadIdx.x] = qsum;
+    reduce_bath(J, I, a0, 1, sR, NN, fs, sU, threadIdx.x, blockDim.x);
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[threadIdx.x].x += red[threadIdx.x + w].x;
+            red[threadIdx.x].y += red[threadIdx.x + w].y;
+        }
+        __syncthreads();
+    }
+    const double2 qn = red[0];
+    for (int p = threadIdx.x; p < d; p += blockDim.x) {
+        const int np_ = J.ado[I.lk_off + (long long)a0 * d + p];
+        if (np_) atomic_add2(acc + 5 * NN + p, make_double2(np_ * qn.x, np_ * qn.y));
+    }
+    for (int t = threadIdx.x; t < 3 * d; t += blockDim.x) {
+        const int w = t / d, p = t - w * d;
+        atomic_add2(acc + 5 * NN + (1 + w) * d + p, make_double2(sU[2 * (w * d + p)], sU[2 * (w * d + p) + 1]));
+    }
+}
+
+struct AdjPrologueArgs {
+    const InstDev* inst;
+    const int* step_ctr;
+    int step_off;
+    const double2* V;   // forward V_i, written by the recompute's prologue
+    double2* VH;
+};
+
+// V_i^H = conj(V_i)^T for the adjoint stages of step i
+__global__ __launch_bounds__(256) void adj_prologue(AdjPrologueArgs P) {
+    const InstDev& I = P.inst[blockIdx.x];
+    const int step = *P.step_ctr + P.step_off;
+    if (step < 0 || step >= I.nsteps) return;
+    const int N = I.N, NN = N * N;
+    for (int q = threadIdx.x; q < NN; q += blockDim.x) {
+        const int a = q / N, b = q - a * N;
+        const double2 v = P.V[I.v_off + b * N + a];
+        P.VH[I.v_off + q] = make_double2(v.x, -v.y);
+    }
+}
+
+struct AdjReadoutArgs {
+    const InstDev* inst;
+    const int* step_ctr;
+    int step_off;
+    const int* pidx;
+    const double2* R;      // readout operators (X3)
+    const double2* gP;     // dL/dP per readout
+    const double2* y;      // y_i (the trajectory slot of this step)
+    double2* lam;
+    double2* gacc;
+    const long long* gacc_off;
+};
+
+// l[ADO 0][j][k] += conj(X3[k][j]) g_i ;  dL/dX3[k][j] += conj(rho_0[j][k]) g_i
+__global__ __launch_bounds__(256) void adj_readout(AdjReadoutArgs P) {
+    const int ii = blockIdx.x;
+    const InstDev& I = P.inst[ii];
+    const int step = *P.step_ctr + P.step_off;
+    if (step < 0 || step >= I.nsteps || I.nread != 1) return;
+    const int slot = P.pidx[I.step_off + step];
+    if (slot < 0) return;
+    const double2 g = P.gP[I.p_off + slot];
+    const int N = I.N, NN = N * N;
+    double2* acc = P.gacc + P.gacc_off[ii] + 4 * NN;
+    for (int q = threadIdx.x; q < NN; q += blockDim.x) {
+        const int j = q / N, k = q - j * N;
+        const double2 x = P.R[I.r_off + k * N + j];
+        double2 l = P.lam[I.state_off + q];
+        const double2 t = cconj_mul(x, g);
+        l.x += t.x;
+        l.y += t.y;
+        P.lam[I.state_off + q] = l;
+        const double2 r = P.y[I.state_off + q];
+        const double2 u = cconj_mul(r, g);
+        acc[k * N + j].x += u.x;
+        acc[k * N + j].y += u.y;
+    }
+}
+
+// y -> checkpoint slot (*ctr / C) + 1, at the end of a C-step forward chunk
+__global__ void ckpt_store(const double2* y, double2* ckpt, long long total, const int* ctr, int C, int nslots) {
+    const int slot = *ctr / C + 1;
+    if (slot >= nslots) return;
+    double2* dst = ckpt + (long long)slot * total;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long long)gridDim.x * blockDim.x)
+        dst[e] = y[e];
+}
+
+// checkpoint slot (*ctr / C) -> dst, at the start of a backward segment
+__global__ void ckpt_load(const double2* ckpt, double2* dst, long long total, const int* ctr, int C) {
+    const double2* src = ckpt + (long long)(*ctr / C) * total;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long long)gridDim.x * blockDim.x)
+        dst[e] = src[e];
+}
+
+__global__ void fill_zero(double2* p, long long n) {
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x)
+        p[e] = make_double2(0.0, 0.0);
+}
+
+__global__ void set_ctr(int* ctr, int v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *ctr = v;
+}
+
+// ---------------------------------------------------------------- host side
+
+struct AdjState {
+    int C = 0;                 // checkpoint interval (steps per segment)
+    int nseg = 0;
+    long long total = 0;       // state elements of the batch
+    DevBuf<double2> ckpt;      // (nseg + 1) states
+    DevBuf<double2> stg;       // (C + 1) x 4 stage inputs of the current segment
+    DevBuf<double2> acc_f;     // forward accumulator scratch of the replay
+    DevBuf<double2> lam, aacc, q1, q2, VH, gP, gacc;
+    DevBuf<int> nbA_up, nbA_dn;
+    DevBuf<double> cxA, cyA;
+    DevBuf<double2> alA, altA, dampA;
+    DevBuf<uint8_t> ado;
+    DevBuf<long long> gacc_off;
+    std::vector<long long> h_gacc_off;
+    hipGraph_t gfwd = nullptr, gbwd = nullptr;
+    hipGraphExec_t xfwd = nullptr, xbwd = nullptr;
+    bool have_grad = false;
+    std::vector<double2> h_gacc;
+    ~AdjState() {
+        if (xfwd) (void)hipGraphExecDestroy(xfwd);
+        if (gfwd) (void)hipGraphDestroy(gfwd);
+        if (xbwd) (void)hipGraphExecDestroy(xbwd);
+        if (gbwd) (void)hipGraphDestroy(gbwd);
+    }
+};
+
+void adj_destroy(AdjState* a) { delete a; }
+
+static int launch_adj(const Group& g, const AdjArgs& base, hipStream_t s) {
+    AdjArgs a = base;
+    a.S.tile_inst += g.tile_begin;
+    a.S.tile_ado0 += g.tile_begin;
+    a.S.tile_nado += g.tile_begin;
+    const dim3 grid(g.tile_count), block(256);
+    switch (g.nt) {
+#define EH_CASE(n) \
+    case n: hipLaunchKernelGGL(adj_small<n>, grid, block, 0, s, a); break;
+        EH_CASE(1) EH_CASE(2) EH_CASE(3) EH_CASE(4) EH_CASE(5) EH_CASE(6) EH_CASE(7) EH_CASE(8)
+        EH_CASE(9) EH_CASE(10) EH_CASE(11) EH_CASE(12) EH_CASE(13) EH_CASE(14) EH_CASE(15) EH_CASE(16)
+#undef EH_CASE
+        default: {
+            // every N > 16 group has one ADO per tile
+            const size_t sh = (size_t)(2 * g.maxnn + g.maxkn) * sizeof(double2) + (size_t)6 * g.maxd * sizeof(double);
+            hipLaunchKernelGGL(adj_generic, grid, block, sh, s, a);
+            break;
+        }
+    }
+    HIPCHK(hipGetLastError());
+    return EH_OK;
+}
+
+int batch_enable_adjoint(Batch* b, int C) {
+    if (!b || !b->impl) return fail(EH_EINVAL, "enable_adjoint: null batch");
+    BatchImpl* im = b->impl;
+    if (im->adj) return fail(EH_EINVAL, "adjoint already enabled");
+    if (C <= 0) return fail(EH_EINVAL, "checkpoint interval must be positive");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    std::unique_ptr<AdjState> a(new AdjState());
+    a->C = std::min(C, std::max(1, b->max_steps));
+    a->nseg = (b->max_steps + a->C - 1) / a->C;
+    a->total = b->total_elems;
+    // adjoint tables: up <- down, down <- up, with the coefficients of the partner link
+    const size_t nq = im->hup.size();
+    std::vector<int> aup(nq), adn(nq);
+    std::vector<double> acx(nq, 0.0), acy(nq, 0.0);
+    std::vector<double2> aal(im->hal.size()), aalt(im->halt.size()), adamp(im->hdamp.size());
+    std::vector<long long> goff;
+    long long gtot = 0;
+    for (int i = 0; i < b->n_inst; i++) {
+        const InstDev& I = im->hinst[(size_t)i];
+        for (int m = 0; m < I.M; m++)
+            for (int p = 0; p < I.d; p++) {
+                const size_t q = (size_t)(I.lk_off + (long long)m * I.d + p);
+                const int j = p % I.K, k = p / I.K;
+                const double2 al = im->hal[(size_t)I.al_off + j * I.Km + k];
+                const double aabs = std::hypot(al.x, al.y);
+                const int mp = im->hado[q];
+                aup[q] = im->hdn[q];
+                adn[q] = im->hup[q];
+                if (aup[q] >= 0) acx[q] = std::sqrt(mp * aabs);          // x of the link (m - e_p) -> m
+                if (adn[q] >= 0) acy[q] = std::sqrt((mp + 1) / aabs);    // y of the link (m + e_p) -> m
+            }
+        goff.push_back(gtot);
+        gtot += gacc_size(I.N * I.N, I.d);
+        if (I.d > 64 && I.N <= 16) return fail(EH_ERANGE, "adjoint: hierarchy dimension above 64");
+    }
+    for (size_t e = 0; e < aal.size(); e++) {
+        aal[e] = make_double2(im->hal[e].x, -im->hal[e].y);
+        aalt[e] = make_double2(im->halt[e].x, -im->halt[e].y);
+    }
+    for (size_t e = 0; e < adamp.size(); e++) adamp[e] = make_double2(im->hdamp[e].x, -im->hdamp[e].y);
+    a->h_gacc_off = goff;
+    int rc;
+    const size_t T = (size_t)a->total;
+    if ((rc = a->nbA_up.upload(aup)) || (rc = a->nbA_dn.upload(adn)) || (rc = a->cxA.upload(acx)) ||
+        (rc = a->cyA.upload(acy)) || (rc = a->alA.upload(aal)) || (rc = a->altA.upload(aalt)) ||
+        (rc = a->dampA.upload(adamp)) || (rc = a->ado.upload(im->hado)) || (rc = a->gacc_off.upload(goff)) ||
+        (rc = a->ckpt.alloc(T * (size_t)(a->nseg + 1))) || (rc = a->stg.alloc(T * 4 * (size_t)(a->C + 1))) ||
+        (rc = a->acc_f.alloc(T)) || (rc = a->lam.alloc(T)) || (rc = a->aacc.alloc(T)) || (rc = a->q1.alloc(T)) ||
+        (rc = a->q2.alloc(T)) || (rc = a->VH.alloc(im->V.n)) || (rc = a->gP.alloc(im->P.n)) ||
+        (rc = a->gacc.alloc((size_t)gtot)))
+        return rc;
+    const int blocks = (int)std::min<long long>(std::max<long long>(1, ((long long)T + 255) / 256), 65536);
+    hipStream_t s = im->stream;
+    StageArgs fw = stage_args(im);
+    PrologueArgs pa = prologue_args(im);
+
+    // forward: C steps + checkpoint store + tick
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < a->C; k++) {
+        pa.step_off = k;
+        hipLaunchKernelGGL(prologue, dim3(b->n_inst), dim3(256), 0, s, pa);
+        for (int st = 0; st < 4; st++) {
+            StageArgs x = fw;
+            x.step_off = k;
+            x.stage = st;
+            x.rin = st == 0 ? im->Y.p : (st == 2 ? im->S2.p : im->S1.p);
+            x.out = st == 3 ? im->Y.p : (st == 1 ? im->S2.p : im->S1.p);
+            for (const Group& g : im->groups)
+                if ((rc = launch_stage(g, x, g.tile_begin, s))) {
+                    hipGraph_t junk;
+                    (void)hipStreamEndCapture(s, &junk);
+                    return rc;
+                }
+        }
+    }
+    hipLaunchKernelGGL(ckpt_store, dim3(blocks), dim3(256), 0, s, im->Y.p, a->ckpt.p, (long long)T, im->step_ctr.p,
+                       a->C, a->nseg + 1);
+    hipLaunchKernelGGL(tick, dim3(1), dim3(1), 0, s, im->step_ctr.p, a->C);
+    HIPCHK(hipStreamEndCapture(s, &a->gfwd));
+    HIPCHK(hipGraphInstantiate(&a->xfwd, a->gfwd, nullptr, nullptr, 0));
+
+    // backward segment: replay from the checkpoint keeping every stage input, then reverse
+    auto stg = [&](int k, int st) { return a->stg.p + ((size_t)k * 4 + st) * T; };
+    AdjArgs aj;
+    std::memset(&aj, 0, sizeof(aj));
+    aj.S = fw;
+    aj.S.nb_up = a->nbA_up.p;
+    aj.S.nb_dn = a->nbA_dn.p;
+    aj.S.cx = a->cxA.p;
+    aj.S.cy = a->cyA.p;
+    aj.S.alpha = a->alA.p;
+    aj.S.alpha_t = a->altA.p;
+    aj.S.damp = a->dampA.p;
+    aj.S.gsign = -1.0;
+    aj.S.V = a->VH.p;
+    aj.fnb_up = im->nb_up.p;
+    aj.fnb_dn = im->nb_dn.p;
+    aj.fcx = im->cx.p;
+    aj.fcy = im->cy.p;
+    aj.ado = a->ado.p;
+    aj.field = im->field.p;
+    aj.gacc = a->gacc.p;
+    aj.gacc_off = a->gacc_off.p;
+    AdjPrologueArgs ap;
+    ap.inst = im->inst.p;
+    ap.step_ctr = im->step_ctr.p;
+    ap.V = im->V.p;
+    ap.VH = a->VH.p;
+    AdjReadoutArgs ar;
+    ar.inst = im->inst.p;
+    ar.step_ctr = im->step_ctr.p;
+    ar.pidx = im->pidx.p;
+    ar.R = im->R.p;
+    ar.gP = a->gP.p;
+    ar.lam = a->lam.p;
+    ar.gacc = a->gacc.p;
+    ar.gacc_off = a->gacc_off.p;
+    pa.readout = 0;   // replays must not overwrite the forward readouts
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(ckpt_load, dim3(blocks), dim3(256), 0, s, a->ckpt.p, stg(0, 0), (long long)T, im->step_ctr.p, a->C);
+    for (int k = 0; k < a->C; k++) {
+        pa.step_off = k;
+        hipLaunchKernelGGL(prologue, dim3(b->n_inst), dim3(256), 0, s, pa);
+        for (int st = 0; st < 4; st++) {
+            StageArgs x = fw;
+            x.step_off = k;
+            x.stage = st;
+            x.y = stg(k, 0);
+            x.acc = a->acc_f.p;
+            x.rin = stg(k, st);
+            x.out = st == 3 ? stg(k + 1, 0) : stg(k, st + 1);
+            for (const Group& g : im->groups)
+                if ((rc = launch_stage(g, x, g.tile_begin, s))) {
+                    hipGraph_t junk;
+                    (void)hipStreamEndCapture(s, &junk);
+                    return rc;
+                }
+        }
+    }
+    for (int k = a->C - 1; k >= 0; k--) {
+        pa.step_off = k;   // V_i again (the replay left V at the segment's last step)
+        hipLaunchKernelGGL(prologue, dim3(b->n_inst), dim3(256), 0, s, pa);
+        ap.step_off = k;
+        hipLaunchKernelGGL(adj_prologue, dim3(b->n_inst), dim3(256), 0, s, ap);
+        for (int st = 0; st < 4; st++) {
+            AdjArgs x = aj;
+            x.S.step_off = k;
+            x.S.stage = st;
+            x.S.y = a->lam.p;
+            x.S.acc = a->aacc.p;
+            x.S.rin = st == 0 ? a->lam.p : (st == 2 ? a->q2.p : a->q1.p);
+            x.S.out = st == 3 ? a->lam.p : (st == 1 ? a->q2.p : a->q1.p);
+            x.Yf = stg(k, 3 - st);
+            for (const Group& g : im->groups)
+                if ((rc = launch_adj(g, x, s))) {
+                    hipGraph_t junk;
+                    (void)hipStreamEndCapture(s, &junk);
+                    return rc;
+                }
+        }
+        ar.step_off = k;
+        ar.y = stg(k, 0);
+        hipLaunchKernelGGL(adj_readout, dim3(b->n_inst), dim3(256), 0, s, ar);
+    }
+    hipLaunchKernelGGL(tick, dim3(1), dim3(1), 0, s, im->step_ctr.p, -a->C);
+    HIPCHK(hipStreamEndCapture(s, &a->gbwd));
+    HIPCHK(hipGraphInstantiate(&a->xbwd, a->gbwd, nullptr, nullptr, 0));
+    im->adj = a.release();
+    return EH_OK;
+}
+
+// forward with checkpoints (called by batch_run when the adjoint is enabled)
+int adj_forward(Batch* b) {
+    BatchImpl* im = b->impl;
+    AdjState* a = im->adj;
+    HIPCHK(hipMemcpyAsync(a->ckpt.p, im->Y.p, (size_t)a->total * sizeof(double2), hipMemcpyDeviceToDevice, im->stream));
+    for (int c = 0; c < a->nseg; c++) HIPCHK(hipGraphLaunch(a->xfwd, im->stream));
+    return EH_OK;
+}
+
+int batch_backward(Batch* b, const double* dP) {
+    if (!b || !b->impl || !b->impl->adj) return fail(EH_EINVAL, "backward: adjoint not enabled");
+    BatchImpl* im = b->impl;
+    AdjState* a = im->adj;
+    if (!im->ran) return fail(EH_EINVAL, "backward before forward run");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    hipStream_t s = im->stream;
+    if (im->P.n) HIPCHK(hipMemcpyAsync(a->gP.p, dP, im->P.n * sizeof(double2), hipMemcpyHostToDevice, s));
+    const long long T = a->total;
+    const int blocks = (int)std::min<long long>(std::max<long long>(1, (T + 255) / 256), 65536);
+    hipLaunchKernelGGL(fill_zero, dim3(blocks), dim3(256), 0, s, a->lam.p, T);
+    hipLaunchKernelGGL(fill_zero, dim3(256), dim3(256), 0, s, a->gacc.p, (long long)a->gacc.n);
+    hipLaunchKernelGGL(set_ctr, dim3(1), dim3(1), 0, s, im->step_ctr.p, (a->nseg - 1) * a->C);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(im->ev0, s));
+    for (int c = 0; c < a->nseg; c++) HIPCHK(hipGraphLaunch(a->xbwd, s));
+    HIPCHK(hipEventRecord(im->ev1, s));
+    HIPCHK(hipStreamSynchronize(s));
+    a->h_gacc.resize(a->gacc.n);
+    HIPCHK(hipMemcpy(a->h_gacc.data(), a->gacc.p, a->gacc.n * sizeof(double2), hipMemcpyDeviceToHost));
+    a->have_grad = true;
+    return EH_OK;
+}
+
+// gradients of instance i in the convention g = dL/dRe + i dL/dIm:
+// dH (N*N), dX4 (4*N*N), dalpha, dalpha_t, dgamma (K*Km each), all complex interleaved
+int batch_get_grad(Batch* b, int i, double* dH, double* dX4, double* dal, double* dalt, double* dga) {
+    if (!b || !b->impl || !b->impl->adj || !b->impl->adj->have_grad) return fail(EH_EINVAL, "no gradient computed");
+    if (i < 0 || i >= b->n_inst) return fail(EH_ERANGE, "instance index out of range");
+    BatchImpl* im = b->impl;
+    AdjState* a = im->adj;
+    const InstDev& I = im->hinst[(size_t)i];
+    const int N = I.N, NN = N * N, d = I.d;
+    const double2* acc = a->h_gacc.data() + a->h_gacc_off[(size_t)i];
+    auto C = [](double2 v) { return cplx(v.x, v.y); };
+    const cplx mi(0, -1);
+    auto put = [](double* o, size_t k, cplx v) {
+        o[2 * k] = v.real();
+        o[2 * k + 1] = v.imag();
+    };
+    // dL/dV = (-i Gt)^H  ->  dH[c][d] = conj(-i Gt[d][c])
+    for (int c = 0; c < N; c++)
+        for (int e = 0; e < N; e++) {
+            if (dH) put(dH, (size_t)c * N + e, std::conj(mi * C(acc[e * N + c])));
+            if (dX4) {
+                // V = H - e1 X[0] - e2 X[1]^T - e3 X[2]^T
+                put(dX4, (size_t)c * N + e, -std::conj(mi * C(acc[NN + e * N + c])));
+                put(dX4, (size_t)NN + (size_t)e * N + c, -std::conj(mi * C(acc[2 * NN + e * N + c])));
+                put(dX4, (size_t)2 * NN + (size_t)e * N + c, -std::conj(mi * C(acc[3 * NN + e * N + c])));
+                put(dX4, (size_t)3 * NN + (size_t)c * N + e, C(acc[4 * NN + c * N + e]));
+            }
+        }
+    const double2* Sg = acc + 5 * NN;
+    const double2* U = Sg + d;
+    const double2* Ap = U + d;
+    const double2* Bp = Ap + d;
+    for (int j = 0; j < I.K; j++)
+        for (int k = 0; k < I.Km; k++) {
+            const int p = j + k * I.K, o = j * I.Km + k;
+            const cplx al = C(im->hal[(size_t)I.al_off + o]), alt = C(im->halt[(size_t)I.al_off + o]);
+            if (dga) put(dga, (size_t)o, -std::conj(C(Sg[p])));
+            const cplx Q = C(U[p]) - al * C(Ap[p]) - alt * C(Bp[p]);
+            if (dal) put(dal, (size_t)o, Q.real() * al / (2.0 * std::norm(al)) + std::conj(C(Ap[p])));
+            if (dalt) put(dalt, (size_t)o, std::conj(C(Bp[p])));
+        }
+    return EH_OK;
+}
+
+}  // namespace eh

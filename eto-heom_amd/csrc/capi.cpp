@@ -284,6 +284,19 @@ int eh_batch_profile(eh_batch* b, int nsteps, double* out5) {
     return b ? batch_profile(b->b, nsteps, out5) : fail(EH_EINVAL, "NULL batch");
 }
 
+int eh_batch_enable_adjoint(eh_batch* b, int checkpoint_every) {
+    return b ? batch_enable_adjoint(b->b, checkpoint_every) : fail(EH_EINVAL, "NULL batch");
+}
+
+int eh_batch_backward(eh_batch* b, const double* dP) {
+    return (b && dP) ? batch_backward(b->b, dP) : fail(EH_EINVAL, "NULL argument");
+}
+
+int eh_batch_get_grad(eh_batch* b, int i, double* dH, double* dX4, double* dalpha, double* dalpha_t,
+                      double* dgamma) {
+    return b ? batch_get_grad(b->b, i, dH, dX4, dalpha, dalpha_t, dgamma) : fail(EH_EINVAL, "NULL batch");
+}
+
 int eh_batch_get_rho(eh_batch* b, int i, double* rho_out) {
     return b ? batch_get_rho(b->b, i, rho_out) : fail(EH_EINVAL, "NULL batch");
 }

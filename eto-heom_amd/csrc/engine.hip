@@ -34,143 +34,11 @@
 #include <string>
 #include <vector>
 
-#include "engine.hpp"
+#include "engine_impl.hpp"
 
 namespace eh {
 
 // ---------------------------------------------------------------- device data
-
-struct InstDev {
-    long long state_off;   // complex offset of ADO 0 in the state buffers
-    long long p_off;       // complex offset in the readout buffer
-    long long lk_off;      // offset of ADO 0's row in the (ADO, mode) neighbour tables
-    int N, M;
-    int ado_off;           // offset in the global ADO numbering (damping)
-    int nsteps;
-    int v_off;             // complex offset of this instance's N*N blocks (H, V, X'_q)
-    int step_off;          // offset in the per-step tables
-    int nread;             // readout operators
-    int npoints;
-    int r_off;             // complex offset of this instance's readout operators
-    int d, K, Km;          // hierarchy dimension d = K*Km, baths, modes per bath
-    int so;                // site offset: bath j couples state j + so
-    int al_off;            // offset of alpha / alpha~ (K*Km complex)
-    double h2, h, hb[4];
-};
-
-struct StageArgs {
-    const InstDev* inst;
-    const int* tile_inst;
-    const int* tile_ado0;
-    const int* tile_nado;
-    const int* step_ctr;
-    int step_off;
-    int stage;
-    const uint8_t* vreal;    // per step: V(t_i) has no imaginary part
-    const double2* V;        // per instance N*N
-    const int* nb_up;        // per (ADO, p): index of n + e_p or -1
-    const int* nb_dn;        // per (ADO, p): index of n - e_p or -1
-    const double* cx;        // per (ADO, p): sqrt((n_p + 1) |alpha_p|)
-    const double* cy;        // per (ADO, p): sqrt(n_p / |alpha_p|)
-    const double2* alpha;    // per instance K*Km
-    const double2* alpha_t;
-    const double2* damp;     // per global ADO: -sum_p n_p gamma_p
-    const double2* rin;      // stage input
-    const double2* y;        // Y
-    double2* acc;            // ACC
-    double2* out;            // stage output
-};
-
-__device__ __forceinline__ double2 cmad(double2 acc, double2 a, double2 b) {
-    acc.x = fma(a.x, b.x, fma(-a.y, b.y, acc.x));
-    acc.y = fma(a.x, b.y, fma(a.y, b.x, acc.y));
-    return acc;
-}
-
-__device__ __forceinline__ double stage_hb(const InstDev& I, int st) {
-    return st == 0 ? I.hb[0] : (st == 1 ? I.hb[1] : (st == 2 ? I.hb[2] : I.hb[3]));
-}
-
-// RK4 stage update of one element from its RHS value k, with y / acc already loaded:
-//   stage 0: ACC = y + h/6 k, out = y + h/2 k;  1: ACC += h/3 k, out = y + h/2 k;
-//   2: ACC += h/3 k, out = y + h k;  3: out (= Y) = ACC + h/6 k
-__device__ __forceinline__ void rk_store(const StageArgs& A, const InstDev& I, size_t gi, double2 y, double2 a,
-                                         double kr, double ki) {
-    const int st = A.stage;
-    const double hb = stage_hb(I, st);
-    if (st == 0) {
-        A.acc[gi] = make_double2(fma(hb, kr, y.x), fma(hb, ki, y.y));
-        A.out[gi] = make_double2(fma(I.h2, kr, y.x), fma(I.h2, ki, y.y));
-    } else if (st < 3) {
-        const double hs = st == 1 ? I.h2 : I.h;
-        A.acc[gi] = make_double2(fma(hb, kr, a.x), fma(hb, ki, a.y));
-        A.out[gi] = make_double2(fma(hs, kr, y.x), fma(hs, ki, y.y));
-    } else {
-        A.out[gi] = make_double2(fma(hb, kr, a.x), fma(hb, ki, a.y));
-    }
-}
-
-// y and acc of element gi as the stage needs them (stage 0 takes y = rin value)
-__device__ __forceinline__ void rk_load(const StageArgs& A, size_t gi, double2 rin_val, double2& y, double2& a) {
-    const int st = A.stage;
-    y = st == 0 ? rin_val : (st < 3 ? A.y[gi] : make_double2(0.0, 0.0));
-    a = st == 0 ? make_double2(0.0, 0.0) : A.acc[gi];
-}
-
-// Coupling gathers of one ADO into LDS.  For bath j (site s = j + so), the
-// reference's one-hot products S_j rho_m and rho_m S_j (dynamics.cpp:143-179)
-// touch only row s and column s, so
-//   cp[(j*2 + 0)*N + b] = sum_k  i x rho_{n+e_p}[s][b] + i alpha y rho_{n-e_p}[s][b]      (row s)
-//   cp[(j*2 + 1)*N + a] = sum_k -i x rho_{n+e_p}[a][s] - i alpha~ y rho_{n-e_p}[a][s]     (column s)
-// with p = j + k K.  Every load is independent, so a thread issues all of its
-// 2 Km gathers before it waits.
-__device__ __forceinline__ void gather_couplings(const StageArgs& A, const InstDev& I, int n0, int na, double2* cp,
-                                                 int tid, int nthr) {
-    const int N = I.N, NN = N * N, K = I.K, Km = I.Km, d = I.d;
-    const int per = 2 * K * N;
-    const int total = per * na;
-    for (int e = tid; e < total; e += nthr) {
-        const int g = e / per, eg = e - g * per;
-        const int j = eg / (2 * N), rem = eg - j * 2 * N;
-        const int rc = rem / N, pos = rem - rc * N;
-        const int s = j + I.so;
-        const int ab = rc == 0 ? s * N + pos : pos * N + s;
-        const long long row0 = I.lk_off + (long long)(n0 + g) * d;
-        double2 acc = make_double2(0.0, 0.0);
-        for (int k = 0; k < Km; ++k) {
-            const long long q = row0 + j + k * K;
-            const int mu = A.nb_up[q], md = A.nb_dn[q];
-            const double2 xu = mu >= 0 ? A.rin[I.state_off + (long long)mu * NN + ab] : make_double2(0.0, 0.0);
-            const double2 xd = md >= 0 ? A.rin[I.state_off + (long long)md * NN + ab] : make_double2(0.0, 0.0);
-            const double x = A.cx[q], y = A.cy[q];
-            const double2 al = rc == 0 ? A.alpha[I.al_off + j * Km + k] : A.alpha_t[I.al_off + j * Km + k];
-            // up:   row  i x,  column -i x;   down: row  i alpha y,  column -i alpha~ y
-            const double sg = rc == 0 ? 1.0 : -1.0;
-            const double2 wu = make_double2(0.0, sg * x);
-            const double2 wd = make_double2(-sg * al.y * y, sg * al.x * y);
-            acc = cmad(acc, wu, xu);
-            acc = cmad(acc, wd, xd);
-        }
-        cp[e] = acc;
-    }
-}
-
-// coupling term of element (row, col) from the gathered rows/columns
-__device__ __forceinline__ double2 coupling_of(const InstDev& I, const double2* cp, int row, int col) {
-    double2 c = make_double2(0.0, 0.0);
-    const int jr = row - I.so, jc = col - I.so;
-    if (jr >= 0 && jr < I.K) {
-        const double2 v = cp[(jr * 2 + 0) * I.N + col];
-        c.x += v.x;
-        c.y += v.y;
-    }
-    if (jc >= 0 && jc < I.K) {
-        const double2 v = cp[(jc * 2 + 1) * I.N + row];
-        c.x += v.x;
-        c.y += v.y;
-    }
-    return c;
-}
 
 // N <= 16: a tile of G = 512/N^2 ADOs per workgroup, two elements per thread.
 template <int NT>
@@ -297,12 +165,6 @@ __global__ __launch_bounds__(256) void stage_generic(StageArgs A) {
     }
 }
 
-typedef double dbl4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ dbl4 mfma4(double a, double b, dbl4 c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-
 // N in (16, NP]: one ADO per workgroup, the commutator on FP64 MFMA 16x16x4 tiles.
 // rho_n (and V when VLDS) are staged in LDS as complex, zero-padded to NP x NP
 // with row stride NP+1 (conflict-free for both operand patterns).  Each of the
@@ -426,72 +288,6 @@ __global__ __launch_bounds__(256) void stage_mfma(StageArgs A) {
     }
 }
 
-struct PrologueArgs {
-    const InstDev* inst;
-    const int* step_ctr;
-    int step_off;
-    const double2* H;        // per instance N*N
-    const double2* Xp;       // per instance 3*N*N: X[0], X[1]^T, X[2]^T
-    const double2* field;    // per step 3 scalars
-    const int* pidx;         // per step readout slot
-    const double2* R;        // per instance nread*N*N readout operators
-    const double2* y;
-    double2* V;
-    double2* P;
-};
-
-// V(t_i) = H - e_1 X[0] - e_2 X[1]^T - e_3 X[2]^T  (dynamics.cpp:348-363), and
-// the readouts Tr(rho_0 R_r) before step i (dynamics.cpp:367-381, :246-253)
-__global__ __launch_bounds__(256) void prologue(PrologueArgs A) {
-    const InstDev& I = A.inst[blockIdx.x];
-    const int step = *A.step_ctr + A.step_off;
-    if (step >= I.nsteps) return;
-    const int N = I.N, NN = N * N;
-    const double2* e = A.field + 3 * (size_t)(I.step_off + step);
-    const double2 e0 = e[0], e1 = e[1], e2 = e[2];
-    const double2* H = A.H + I.v_off;
-    const double2* X = A.Xp + 3 * (size_t)I.v_off;
-    double2* V = A.V + I.v_off;
-    for (int q = threadIdx.x; q < NN; q += blockDim.x) {
-        double2 v = H[q];
-        const double2 x0 = X[q], x1 = X[NN + q], x2 = X[2 * NN + q];
-        v.x -= x0.x * e0.x - x0.y * e0.y;
-        v.y -= x0.x * e0.y + x0.y * e0.x;
-        v.x -= x1.x * e1.x - x1.y * e1.y;
-        v.y -= x1.x * e1.y + x1.y * e1.x;
-        v.x -= x2.x * e2.x - x2.y * e2.y;
-        v.y -= x2.x * e2.y + x2.y * e2.x;
-        V[q] = v;
-    }
-    const int slot = A.pidx[I.step_off + step];
-    if (slot < 0) return;
-    __shared__ double2 red[256];
-    const double2* rho0 = A.y + I.state_off;
-    for (int r = 0; r < I.nread; ++r) {
-        const double2* R = A.R + I.r_off + (size_t)r * NN;
-        double2 s = make_double2(0.0, 0.0);
-        for (int q = threadIdx.x; q < NN; q += blockDim.x) {
-            const int j = q / N, k = q - j * N;
-            s = cmad(s, rho0[q], R[k * N + j]);
-        }
-        red[threadIdx.x] = s;
-        __syncthreads();
-        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) {
-                red[threadIdx.x].x += red[threadIdx.x + w].x;
-                red[threadIdx.x].y += red[threadIdx.x + w].y;
-            }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) A.P[I.p_off + (long long)r * I.npoints + slot] = red[0];
-        __syncthreads();
-    }
-}
-
-__global__ void tick(int* ctr, int by) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *ctr += by;
-}
-
 // rho_0 = |0><0| for every instance, all other ADOs zero; step counter to 0
 __global__ void reset_state(double2* y, long long total, const long long* ado0_off, int n_inst, int* ctr) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -510,73 +306,6 @@ __global__ void set_ground(double2* y, const long long* ado0_off, int n_inst) {
 
 namespace eh {
 
-namespace {
-
-#define HIPCHK(expr)                                                                   \
-    do {                                                                               \
-        hipError_t e_ = (expr);                                                        \
-        if (e_ != hipSuccess)                                                          \
-            return fail(EH_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
-    } while (0)
-
-template <typename T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
-    int alloc(size_t count) {
-        n = count;
-        if (count == 0) count = 1;
-        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
-        if (e != hipSuccess) {
-            p = nullptr;
-            return fail(e == hipErrorOutOfMemory ? EH_ENOMEM : EH_EHIP,
-                        std::string("hipMalloc: ") + hipGetErrorString(e));
-        }
-        return EH_OK;
-    }
-    int upload(const std::vector<T>& h) {
-        int rc = alloc(h.size());
-        if (rc) return rc;
-        if (!h.empty()) HIPCHK(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
-        return EH_OK;
-    }
-};
-
-struct Group {
-    int nt;           // compile-time N of stage_small, 0 = stage_generic
-    int tile_begin, tile_count;
-    size_t shmem;
-};
-
-}  // namespace
-
-struct BatchImpl {
-    hipStream_t stream = nullptr;
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t graph = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    DevBuf<InstDev> inst;
-    DevBuf<int> tile_inst, tile_ado0, tile_nado, step_ctr, nb_up, nb_dn, pidx;
-    DevBuf<uint8_t> vreal;
-    DevBuf<double> cx, cy;
-    DevBuf<double2> V, H, Xp, field, R, P, alpha, alpha_t, damp, Y, ACC, S1, S2;
-    DevBuf<long long> ado0_off;
-    std::vector<Group> groups;
-    int nchunks = 0;
-    bool ran = false;
-
-    ~BatchImpl() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        if (ev0) (void)hipEventDestroy(ev0);
-        if (ev1) (void)hipEventDestroy(ev1);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
-
 int ctx_init(int device, eh_ctx* ctx) {
     int count = 0;
     hipError_t e = hipGetDeviceCount(&count);
@@ -594,7 +323,7 @@ int ctx_init(int device, eh_ctx* ctx) {
 
 static inline double2 d2(cplx c) { return make_double2(c.real(), c.imag()); }
 
-static int launch_stage(const Group& g, const StageArgs& base, int tile0, hipStream_t s) {
+int launch_stage(const Group& g, const StageArgs& base, int tile0, hipStream_t s) {
     StageArgs a = base;
     a.tile_inst += tile0;
     a.tile_ado0 += tile0;
@@ -614,7 +343,7 @@ static int launch_stage(const Group& g, const StageArgs& base, int tile0, hipStr
     return EH_OK;
 }
 
-static StageArgs stage_args(const BatchImpl* im) {
+StageArgs stage_args(const BatchImpl* im) {
     StageArgs sa;
     std::memset(&sa, 0, sizeof(sa));
     sa.inst = im->inst.p;
@@ -631,12 +360,13 @@ static StageArgs stage_args(const BatchImpl* im) {
     sa.alpha = im->alpha.p;
     sa.alpha_t = im->alpha_t.p;
     sa.damp = im->damp.p;
+    sa.gsign = 1.0;
     sa.y = im->Y.p;
     sa.acc = im->ACC.p;
     return sa;
 }
 
-static PrologueArgs prologue_args(const BatchImpl* im) {
+PrologueArgs prologue_args(const BatchImpl* im) {
     PrologueArgs pa;
     std::memset(&pa, 0, sizeof(pa));
     pa.inst = im->inst.p;
@@ -649,6 +379,7 @@ static PrologueArgs prologue_args(const BatchImpl* im) {
     pa.y = im->Y.p;
     pa.V = im->V.p;
     pa.P = im->P.p;
+    pa.readout = 1;
     return pa;
 }
 
@@ -664,6 +395,7 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
     std::vector<InstDev> hinst(n);
     std::vector<int> t_inst, t_ado0, t_nado;
     std::vector<int> hup, hdn, hpidx;
+    std::vector<uint8_t> hado;
     std::vector<double> hcx, hcy;
     std::vector<uint8_t> hvreal;
     std::vector<double2> hH, hXp, hfield, hR, hal, halt, hdamp;
@@ -732,6 +464,7 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
         }
         for (int m = 0; m < M; m++) hdamp.push_back(d2(tb.damp[(size_t)m]));
         hup.insert(hup.end(), tb.up.begin(), tb.up.end());
+        hado.insert(hado.end(), h.ado.begin(), h.ado.end());
         hdn.insert(hdn.end(), tb.dn.begin(), tb.dn.end());
         hcx.insert(hcx.end(), tb.cx.begin(), tb.cx.end());
         hcy.insert(hcy.end(), tb.cy.begin(), tb.cy.end());
@@ -799,7 +532,7 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
         g.nt = nt <= 16 ? nt : (nt == 17 ? -32 : -64);
         g.tile_begin = (int)t_inst.size();
         g.tile_count = (int)(tv.size() / 3);
-        int maxnn = 0, maxkn = 0;
+        int maxnn = 0, maxkn = 0, maxd = 0;
         for (size_t k = 0; k < tv.size(); k += 3) {
             t_inst.push_back(tv[k]);
             t_ado0.push_back(tv[k + 1]);
@@ -807,8 +540,12 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
             const InstDev& di = hinst[(size_t)tv[k]];
             maxnn = std::max(maxnn, di.N * di.N);
             maxkn = std::max(maxkn, 2 * di.K * di.N);
+            maxd = std::max(maxd, di.d);
         }
         const size_t cpb = (size_t)maxkn * sizeof(double2);   // coupling rows/columns
+        g.maxnn = maxnn;
+        g.maxkn = maxkn;
+        g.maxd = maxd;
         if (g.nt == -32)
             g.shmem = (size_t)2 * 32 * 33 * sizeof(double2) + cpb;
         else if (g.nt == -64)
@@ -839,6 +576,15 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
         (rc = im->Y.alloc((size_t)state_off)) || (rc = im->ACC.alloc((size_t)state_off)) ||
         (rc = im->S1.alloc((size_t)state_off)) || (rc = im->S2.alloc((size_t)state_off)))
         return rc;
+    im->hinst = hinst;
+    im->hup = std::move(hup);
+    im->hdn = std::move(hdn);
+    im->hcx = std::move(hcx);
+    im->hcy = std::move(hcy);
+    im->hal = std::move(hal);
+    im->halt = std::move(halt);
+    im->hdamp = std::move(hdamp);
+    im->hado = std::move(hado);
     HIPCHK(hipStreamCreateWithFlags(&im->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&im->ev0));
     HIPCHK(hipEventCreate(&im->ev1));
@@ -899,7 +645,12 @@ int batch_run(Batch* b) {
                        im->ado0_off.p, b->n_inst);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(im->ev0, im->stream));
-    for (int c = 0; c < im->nchunks; c++) HIPCHK(hipGraphLaunch(im->exec, im->stream));
+    if (im->adj) {
+        const int rc = adj_forward(b);
+        if (rc) return rc;
+    } else {
+        for (int c = 0; c < im->nchunks; c++) HIPCHK(hipGraphLaunch(im->exec, im->stream));
+    }
     HIPCHK(hipEventRecord(im->ev1, im->stream));
     im->ran = true;
     b->synced = false;

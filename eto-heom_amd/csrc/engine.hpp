@@ -41,6 +41,9 @@ int batch_sync(Batch* b);
 int batch_elapsed_ms(Batch* b, double* ms);
 int batch_get_rho(Batch* b, int i, double* rho_out);
 int batch_profile(Batch* b, int nsteps, double* out5);
+int batch_enable_adjoint(Batch* b, int checkpoint_every);
+int batch_backward(Batch* b, const double* dP);
+int batch_get_grad(Batch* b, int i, double* dH, double* dX4, double* dalpha, double* dalpha_t, double* dgamma);
 int ctx_init(int device, eh_ctx* ctx);
 
 }  // namespace eh

@@ -1,0 +1,100 @@
+// engine_impl.hpp -- host-side internals of the batch engine shared by engine.hip
+// and adjoint.hip (device buffers, kernel groups, the per-batch state).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+#include "engine.hpp"
+
+namespace eh {
+
+#define HIPCHK(expr)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(EH_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t count) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = count;
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(e == hipErrorOutOfMemory ? EH_ENOMEM : EH_EHIP,
+                        std::string("hipMalloc: ") + hipGetErrorString(e));
+        }
+        return EH_OK;
+    }
+    int upload(const std::vector<T>& h) {
+        int rc = alloc(h.size());
+        if (rc) return rc;
+        if (!h.empty()) HIPCHK(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+        return EH_OK;
+    }
+};
+
+struct Group {
+    int nt;           // 1..16: stage_small<nt>; -32 / -64: stage_mfma; 0: stage_generic
+    int tile_begin, tile_count;
+    size_t shmem;
+    int maxnn, maxkn, maxd;   // largest N^2, 2 K N and d of the group's instances
+};
+
+struct AdjState;   // adjoint.hip
+void adj_destroy(AdjState* a);
+
+struct BatchImpl {
+    hipStream_t stream = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevBuf<InstDev> inst;
+    DevBuf<int> tile_inst, tile_ado0, tile_nado, step_ctr, nb_up, nb_dn, pidx;
+    DevBuf<uint8_t> vreal;
+    DevBuf<double> cx, cy;
+    DevBuf<double2> V, H, Xp, field, R, P, alpha, alpha_t, damp, Y, ACC, S1, S2;
+    DevBuf<long long> ado0_off;
+    std::vector<Group> groups;
+    int nchunks = 0;
+    bool ran = false;
+    // host copies the adjoint needs to build its tables
+    std::vector<InstDev> hinst;
+    std::vector<int> hup, hdn;
+    std::vector<double> hcx, hcy;
+    std::vector<double2> hal, halt, hdamp;
+    std::vector<uint8_t> hado;           // multi-indices, same (ADO, p) layout as the tables
+    AdjState* adj = nullptr;
+
+    ~BatchImpl() {
+        if (adj) adj_destroy(adj);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+int launch_stage(const Group& g, const StageArgs& base, int tile0, hipStream_t s);
+int adj_forward(Batch* b);
+StageArgs stage_args(const BatchImpl* im);
+PrologueArgs prologue_args(const BatchImpl* im);
+
+}  // namespace eh

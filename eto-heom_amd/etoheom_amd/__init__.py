@@ -41,7 +41,8 @@ EXPORTS = [
     "eh_ctx_create", "eh_ctx_destroy", "eh_ctx_info", "eh_2des_solve", "eh_dynamics_solve",
     "eh_2des_key_run", "eh_dynamics_key_run", "eh_batch_create", "eh_batch_destroy", "eh_batch_run",
     "eh_batch_sync", "eh_batch_elapsed_ms", "eh_batch_get_output", "eh_batch_get_rho", "eh_batch_dims",
-    "eh_batch_ado_updates", "eh_batch_profile",
+    "eh_batch_ado_updates", "eh_batch_profile", "eh_batch_enable_adjoint", "eh_batch_backward",
+    "eh_batch_get_grad",
 ]
 
 
@@ -112,6 +113,9 @@ def lib():
             "eh_batch_dims": (C.c_int, [VP, VP]),
             "eh_batch_ado_updates": (I64, [VP]),
             "eh_batch_profile": (C.c_int, [VP, C.c_int, VP]),
+            "eh_batch_enable_adjoint": (C.c_int, [VP, C.c_int]),
+            "eh_batch_backward": (C.c_int, [VP, VP]),
+            "eh_batch_get_grad": (C.c_int, [VP, C.c_int, VP, VP, VP, VP, VP]),
         }
         for name, (res, args) in sig.items():
             f = getattr(l, name)
@@ -365,7 +369,8 @@ class Batch:
     """Independent propagations run together: items are dicts with keys
     problem (Problem), H, X (2DES), nv, mode, sites (dynamics)."""
 
-    def __init__(self, ctx: Context, items):
+    def __init__(self, ctx: Context, items, adjoint: int = 0):
+        """adjoint > 0 enables gradients with a checkpoint every `adjoint` steps."""
         self.ctx = ctx
         self._keep = []
         arr = (Instance * len(items))()
@@ -393,6 +398,25 @@ class Batch:
         _check(lib().eh_batch_dims(self._h, d.ctypes.data))
         self.n_inst, self.total_ados, self.total_elems, self.max_steps, self.chunk = (int(v) for v in d)
         self.ado_updates = int(lib().eh_batch_ado_updates(self._h))
+        if adjoint:
+            _check(lib().eh_batch_enable_adjoint(self._h, int(adjoint)))
+
+    def backward(self, dP):
+        """dP: list of complex arrays (dL/dRe P + i dL/dIm P), one per instance."""
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(g, dtype=np.complex128).ravel() for g in dP]))
+        _check(lib().eh_batch_backward(self._h, flat.ctypes.data))
+
+    def grad(self, i: int):
+        """Gradients of instance i: dict H (N,N), X (4,N,N), alpha, alpha_t, gamma (K,Km)."""
+        p = self.items[i]["problem"]
+        N, (K, Km) = p.N, p.alpha.shape
+        out = {"H": np.zeros((N, N), complex), "X": np.zeros((4, N, N), complex),
+               "alpha": np.zeros((K, Km), complex), "alpha_t": np.zeros((K, Km), complex),
+               "gamma": np.zeros((K, Km), complex)}
+        _check(lib().eh_batch_get_grad(self._h, i, out["H"].ctypes.data, out["X"].ctypes.data,
+                                       out["alpha"].ctypes.data, out["alpha_t"].ctypes.data,
+                                       out["gamma"].ctypes.data))
+        return out
 
     def run(self):
         _check(lib().eh_batch_run(self._h))

@@ -56,7 +56,7 @@ def key_text(single_size, K, L, H_cm, dipoles, modes, lambda_line, pulses, time,
     out += ["", "BATH", repr(float(lambda_line)), str(len(modes))]
     out += [" ".join(repr(float(v)) for v in m) for m in modes]
     out += ["", "DIPOLE", str(len(dipoles))]
-    out += [f"{i} {j} {x!r} {y!r} {z!r}" for i, j, x, y, z in dipoles]
+    out += [f"{int(i)} {int(j)} {float(x)!r} {float(y)!r} {float(z)!r}" for i, j, x, y, z in dipoles]
     out += ["", "POLARIZATION", " ".join(str(v) for v in polarization), "", "PULSE", "3"]
     out += [" ".join(repr(float(v)) for v in p) for p in pulses]
     out += ["TIME", " ".join(repr(float(v)) for v in time), "", "DISORDER", f"{n_sample} {seed}"]

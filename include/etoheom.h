@@ -156,6 +156,21 @@ int eh_batch_dims(const eh_batch* b, int64_t* out5);
 /* per-kernel timing pass: nsteps steps launched without the graph, HIP events
  * around each launch on the batch stream; out5 = mean ms of {prologue, stage 0..3} */
 int eh_batch_profile(eh_batch* b, int nsteps, double* out5);
+/* ---- gradients (discrete adjoint of the RK4 propagation) -------------------
+ * No reference counterpart (the reference has no derivatives); this is the
+ * "adjoint gradient" of the propagation path.  After enable, eh_batch_run stores
+ * the state every `checkpoint_every` steps; eh_batch_backward takes
+ * dP = dL/dRe P + i dL/dIm P for every 2DES readout (instances concatenated in
+ * order, eh_batch_get_output's point count each), replays each checkpoint
+ * segment and runs the transposed hierarchy backwards.  eh_batch_get_grad then
+ * returns dL/dH (N*N), dL/dX (4*N*N, X[0..3] as passed in), dL/dalpha,
+ * dL/dalpha_t, dL/dgamma (K*Km) of instance i, same convention, complex
+ * interleaved; any output pointer may be NULL. */
+int eh_batch_enable_adjoint(eh_batch* b, int checkpoint_every);
+int eh_batch_backward(eh_batch* b, const double* dP);
+int eh_batch_get_grad(eh_batch* b, int i, double* dH, double* dX4, double* dalpha, double* dalpha_t,
+                      double* dgamma);
+
 /* ADO-updates per run (sum over instances of M * steps) */
 int64_t eh_batch_ado_updates(const eh_batch* b);
 

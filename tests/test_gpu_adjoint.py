@@ -1,0 +1,132 @@
+"""GPU adjoint gradients against central finite differences of the CPU oracle.
+
+Loss: L = Re sum_t conj(w_t) P_t for fixed random weights w, so dL/dP = w in the
+convention g = dL/dRe P + i dL/dIm P; each checked parameter theta is perturbed
+by +-h in the oracle (relative step 1e-4 unless the value is zero) and
+dL/dtheta from the adjoint must agree to <= 1e-6 relative (|FD| scale).
+"""
+import numpy as np
+import pytest
+
+import etoheom_amd as E
+import oracle as O
+from etoheom_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return E.Context(0)
+
+
+def oproblem(p, **over):
+    kw = dict(H=p.H, X=p.X, alpha=p.alpha, alpha_t=p.alpha_t, gamma=p.gamma, L=p.L, site_offset=p.site_offset,
+              pulses=p.pulses, t_start=p.t_start, t_end=p.t_end, step_size=p.step_size, print_step=p.print_step)
+    kw.update(over)
+    return O.Problem(**kw)
+
+
+def loss(q, w):
+    P, _ = O.twod(q, nv=(1, 1, 1))
+    return float(np.real(np.sum(np.conj(w) * P)))
+
+
+def fd(p, w, name, idx, part):
+    """central FD of the oracle loss wrt Re/Im of one entry of array `name`."""
+    base = getattr(p, name).astype(np.complex128)
+    v = base[idx]
+    mag = abs(v.real if part == "re" else v.imag)
+    h = 1e-4 * mag if mag > 0 else 1e-6
+    step = h if part == "re" else 1j * h
+    a, b = base.copy(), base.copy()
+    a[idx] += step
+    b[idx] -= step
+    return (loss(oproblem(p, **{name: a}), w) - loss(oproblem(p, **{name: b}), w)) / (2 * h)
+
+
+def adj_value(g, name, idx, part):
+    v = g[name][idx]
+    return v.real if part == "re" else v.imag
+
+
+CHECKS = [("H", (1, 1), "re"), ("H", (1, 2), "re"), ("H", (3, 3), "re"), ("X", (0, 0, 1), "re"),
+          ("X", (1, 0, 2), "re"), ("X", (2, 1, 3), "re"), ("X", (3, 1, 0), "re"), ("gamma", (0, 0), "re"),
+          ("gamma", (1, 1), "im"), ("alpha", (0, 1), "re"), ("alpha", (1, 0), "im"), ("alpha_t", (0, 0), "im")]
+
+
+def small_dimer():
+    p = configs.dimer(L=2, nsteps=160, t1=10.0, t2=10.0)
+    p.pulses[:, 1] = [20.0, 30.0, 40.0]
+    p.pulses[:, 2] = 8.0
+    p.pulses[:, 0] *= 3e5        # a strong field so that every term matters
+    p.gamma = p.gamma * 4.0
+    p.alpha = p.alpha * 8.0
+    p.alpha_t = p.alpha_t * 8.0
+    return p
+
+
+@pytest.mark.parametrize("C", [160, 7])
+def test_adjoint_matches_oracle_fd_dimer(ctx, C):
+    p = small_dimer()
+    P0, _ = O.twod(oproblem(p), nv=(1, 1, 1))
+    rng = np.random.default_rng(1)
+    w = rng.normal(size=len(P0)) + 1j * rng.normal(size=len(P0))
+    b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)}], adjoint=C)
+    b.run()
+    b.sync()
+    P = b.output(0)
+    assert np.max(np.abs(P - P0)) / np.max(np.abs(P0)) < 1e-10
+    b.backward([w])
+    g = b.grad(0)
+    for name, idx, part in CHECKS:
+        ref = fd(p, w, name, idx, part)
+        got = adj_value(g, name, idx, part)
+        scale = max(abs(ref), 1e-3 * np.max(np.abs(P0)) * np.max(np.abs(w)))
+        assert abs(got - ref) / scale < 1e-6, (name, idx, part, got, ref)
+
+
+def test_adjoint_batch_of_two_sizes(ctx):
+    """Per-instance offsets: a dimer and a trimer in one batch, gradients of each vs FD."""
+    p1 = small_dimer()
+    p2 = configs.aggregate(3, 2, nsteps=120, taus=(15.0, 25.0, 35.0))
+    p2.pulses[:, 2] = 8.0
+    p2.pulses[:, 0] *= 3e5
+    items = [p1, p2]
+    ws = []
+    rng = np.random.default_rng(2)
+    for p in items:
+        P0, _ = O.twod(oproblem(p), nv=(1, 1, 1))
+        ws.append(rng.normal(size=len(P0)) + 1j * rng.normal(size=len(P0)))
+    b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)} for p in items], adjoint=16)
+    b.run()
+    b.sync()
+    b.backward(ws)
+    for i, p in enumerate(items):
+        g = b.grad(i)
+        for name, idx, part in [("H", (2, 2), "re"), ("X", (0, 0, 2), "re"), ("gamma", (1, 0), "re"),
+                                ("alpha", (0, 0), "re")]:
+            ref = fd(p, ws[i], name, idx, part)
+            got = adj_value(g, name, idx, part)
+            assert abs(got - ref) / max(abs(ref), 1e-12) < 1e-6, (i, name, idx, got, ref)
+
+
+def test_adjoint_generic_large_n(ctx):
+    """N = 29 goes through the generic adjoint kernel (MFMA forward)."""
+    p = configs.heptamer(L=1, nsteps=40)
+    p.pulses[:, 1] = [2.0, 6.0, 10.0]
+    p.pulses[:, 2] = 3.0
+    p.pulses[:, 0] *= 3e5
+    P0, _ = O.twod(oproblem(p), nv=(1, 1, 1))
+    rng = np.random.default_rng(3)
+    w = rng.normal(size=len(P0)) + 1j * rng.normal(size=len(P0))
+    b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)}], adjoint=8)
+    b.run()
+    b.sync()
+    b.backward([w])
+    g = b.grad(0)
+    for name, idx, part in [("H", (1, 1), "re"), ("H", (2, 9), "re"), ("X", (0, 0, 3), "re"),
+                            ("gamma", (2, 1), "re"), ("alpha", (4, 0), "re")]:
+        ref = fd(p, w, name, idx, part)
+        got = adj_value(g, name, idx, part)
+        assert abs(got - ref) / max(abs(ref), 1e-12) < 1e-6, (name, idx, got, ref)
