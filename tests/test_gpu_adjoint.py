@@ -2,8 +2,8 @@
 
 Loss: L = Re sum_t conj(w_t) P_t for fixed random weights w, so dL/dP = w in the
 convention g = dL/dRe P + i dL/dIm P; each checked parameter theta is perturbed
-by +-h in the oracle (relative step 1e-4 unless the value is zero) and
-dL/dtheta from the adjoint must agree to <= 1e-6 relative (|FD| scale).
+by +-h in the oracle (relative steps in REL_STEP, Richardson-extrapolated with
+h/2) and dL/dtheta from the adjoint must agree to <= 1e-6 relative (|FD| scale).
 """
 import numpy as np
 import pytest
@@ -32,17 +32,31 @@ def loss(q, w):
     return float(np.real(np.sum(np.conj(w) * P)))
 
 
+# relative FD steps: 1e-4 for H, whose entries multiply t in the phase (a plain
+# central difference then errs by ~(h t)^2 ~ 1e-5, hence the Richardson step);
+# 1e-3 for X and gamma; 1e-1 for alpha, alpha~ (|alpha| ~ 3e-4, so smaller steps
+# leave the difference dominated by rounding, eps |L| / h ~ 1e-6 of the derivative,
+# while the Richardson value is already converged to ~1e-8 at this step)
+REL_STEP = {"H": 1e-4, "X": 1e-3, "alpha": 1e-1, "alpha_t": 1e-1, "gamma": 1e-3}
+
+
 def fd(p, w, name, idx, part):
-    """central FD of the oracle loss wrt Re/Im of one entry of array `name`."""
+    """Central FD of the oracle loss wrt Re/Im of one entry of array `name`,
+    Richardson-extrapolated from steps h and h/2."""
     base = getattr(p, name).astype(np.complex128)
     v = base[idx]
     mag = abs(v.real if part == "re" else v.imag)
-    h = 1e-4 * mag if mag > 0 else 1e-6
-    step = h if part == "re" else 1j * h
-    a, b = base.copy(), base.copy()
-    a[idx] += step
-    b[idx] -= step
-    return (loss(oproblem(p, **{name: a}), w) - loss(oproblem(p, **{name: b}), w)) / (2 * h)
+    # relative to the entry, or to the array's scale for (near-)zero entries
+    h0 = REL_STEP[name] * max(mag, 0.5 * np.max(np.abs(base)))
+
+    def central(h):
+        step = h if part == "re" else 1j * h
+        a, b = base.copy(), base.copy()
+        a[idx] += step
+        b[idx] -= step
+        return (loss(oproblem(p, **{name: a}), w) - loss(oproblem(p, **{name: b}), w)) / (2 * h)
+
+    return (4 * central(h0 / 2) - central(h0)) / 3
 
 
 def adj_value(g, name, idx, part):
