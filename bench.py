@@ -38,6 +38,9 @@ def workload(name, nsteps=None):
         return [configs.aggregate(3, 5)], "trimer N=7, d=6, L=5, 462 ADOs, 2000 RK4 steps"
     if name == "chain10":
         return [configs.chain()], "10-site chain N=56, d=20, L=3, 1771 ADOs, 3000 RK4 steps"
+    if name == "trimer_grad":
+        return None, ("trimer N=7 (ground+3+3), d=6, L=5, 462 ADOs: 16x16 (t1, t3) rephasing grid at t2=100 fs, "
+                      "48 propagations, loss gradient by the adjoint (checkpoint every 32 steps)")
     if name == "grid2d":
         return configs.grid2d(), "512 propagations N=4/7/11, L=3..6, static disorder, 2000 RK4 steps each"
     raise SystemExit(f"unknown workload {name}")
@@ -78,6 +81,48 @@ def traffic_of(workload):
     return d["traffic_bytes"], d["source"]
 
 
+def bench_gradient(a):
+    """Forward + adjoint of the trimer 2DES-grid loss; reports the cost ratio
+    (forward + backward) / forward and the equivalent central-FD cost."""
+    import numpy as np
+    import torch
+    import etoheom_amd as E
+    from etoheom_amd import spectra as SP
+    ctx = E.Context(0)
+    spec = SP.grid_spec(ctx)
+    with torch.no_grad():
+        target = SP.grid_signal(SP.trimer_params(perturb=5), spec)
+    fw, bw = [], []
+    for it in range(a.warmup + a.steps):
+        params = SP.trimer_params()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        S = SP.grid_signal(params, spec)
+        loss = SP.grid_loss(S, target)
+        t1 = time.perf_counter()
+        loss.backward()
+        t2 = time.perf_counter()
+        if it >= a.warmup:
+            fw.append(t1 - t0)
+            bw.append(t2 - t1)
+    n_params = sum(p.numel() for p in params.values())
+    f, b = float(np.mean(fw)), float(np.mean(bw))
+    # ADO-updates of the forward: 48 propagations x 462 ADOs x their steps
+    steps = sum(int(np.floor(np.float32(np.float32(te) / np.float32(spec.step_size)))) + 1 for te in spec.t_end)
+    ups = 462 * steps
+    rec = {"metric": "FP64 ADO-updates/s + 2DES grid points/s at 1e-10 err; adjoint-grad time vs FD",
+           "value": ups / f, "unit": "ADO-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": (f + b) * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": workload("trimer_grad")[1], "device": ctx.name,
+                      "grid_points_per_s": 16 / f,
+                      "adjoint": {"forward_s": f, "backward_s": b, "gradient_over_forward": (f + b) / f,
+                                  "n_params": n_params, "central_fd_s": 2 * n_params * f,
+                                  "fd_over_adjoint": 2 * n_params * f / (f + b), "loss": float(loss.detach())}}}
+    print(json.dumps(rec))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +151,8 @@ def main():
             torch.mul(x, 1.0, out=y)
         torch.cuda.synchronize()
         del x, y
+    if a.workload == "trimer_grad":
+        return bench_gradient(a)
     items, desc = workload(a.workload, a.nsteps)
     ctx = E.Context(0)
     batch = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)} for p in items])

@@ -207,3 +207,27 @@ def test_samples_are_physical(tmp_path):
     nrm = np.abs(X[:, 0, 0, 1]) ** 2 + np.abs(X[:, 0, 0, 2]) ** 2
     assert np.all(nrm <= 1 + 1e-12)
     assert np.all(np.abs(X[:, 0]) <= 1 + 1e-12)
+
+
+def test_torch_system_builder_matches_key_path(tmp_path):
+    """The differentiable H/X construction equals the key parser + construct_Hal + dipole path."""
+    import torch
+    from etoheom_amd import autograd as AG, configs
+    rng = np.random.default_rng(11)
+    n = 3
+    eps = np.array([12400.0, 12150.0, 12300.0])
+    J = rng.uniform(-100, 100, (n, n))
+    J = np.triu(J, 1) + np.triu(J, 1).T
+    mu = rng.normal(size=(n, 3))
+    lam = configs.OHMIC_35_100_77["lambda_line"]
+    txt = configs.key_text(n + 1, n, 1, configs._sites_H(eps, J), [(0, i + 1, *mu[i]) for i in range(n)],
+                           configs.ohmic_modes(), lam, [(1, 10, 5, 1)] * 3, (0, 10, 1, 1),
+                           polarization=(0, 90, 0, 90))
+    f = tmp_path / "t.key"
+    f.write_text(txt)
+    k = E.Param.from_key(str(f))
+    pol = k.polar_default()
+    k.set_polar(pol)
+    H, X = AG.build_2des_system(torch.tensor(eps), torch.tensor(J), torch.tensor(mu), lam, pol)
+    assert np.allclose(H.numpy(), k.construct_hal(), rtol=0, atol=1e-15)
+    assert np.allclose(X.numpy(), k.get("X"), rtol=0, atol=1e-15)
