@@ -175,14 +175,17 @@ __global__ __launch_bounds__(256) void stage_generic(StageArgs A) {
 // All global loads of the workgroup (rho_n, V, coupling gathers, and y / acc of
 // the lane's output elements) are issued before the first wait.
 template <int NP, bool VLDS>
-__global__ __launch_bounds__(256) void stage_mfma(StageArgs A) {
+__global__ __launch_bounds__(256, NP == 32 ? 4 : 1) void stage_mfma(StageArgs A) {
     constexpr int LD = NP + 1;
     constexpr int NB = NP / 16;
     constexpr int TPW = NB * NB / 4;
     constexpr int SPT = NP * NP / 256;   // staged entries per thread
     extern __shared__ double2 smem[];
     double2* sR = smem;
-    double2* sV = smem + NP * LD;
+    // V as real and imaginary planes of doubles: the 8-byte operand reads of the
+    // commutator then walk a 33-double row stride without bank conflicts
+    double* sVr = reinterpret_cast<double*>(smem + NP * LD);
+    double* sVi = sVr + NP * LD;
     double2* sC = smem + (VLDS ? 2 : 1) * NP * LD;
     const int tile = blockIdx.x;
     const InstDev& I = A.inst[A.tile_inst[tile]];
@@ -228,13 +231,16 @@ __global__ __launch_bounds__(256) void stage_mfma(StageArgs A) {
         const int e = threadIdx.x + q * 256;
         const int r = e / NP, c = e - r * NP;
         sR[r * LD + c] = xr[q];
-        if (VLDS) sV[r * LD + c] = xv[q];
+        if (VLDS) {
+            sVr[r * LD + c] = xv[q].x;
+            sVi[r * LD + c] = xv[q].y;
+        }
     }
     __syncthreads();
     const bool vreal = A.vreal[I.step_off + step] != 0;
     const double2 g = A.damp[I.ado_off + a0];
     auto vload = [&](int r, int c) -> double2 {
-        if (VLDS) return sV[r * LD + c];
+        if (VLDS) return make_double2(sVr[r * LD + c], sVi[r * LD + c]);
         return (r < N && c < N) ? V[r * N + c] : make_double2(0.0, 0.0);
     };
 #pragma unroll
@@ -289,6 +295,312 @@ __global__ __launch_bounds__(256) void stage_mfma(StageArgs A) {
 }
 
 // rho_0 = |0><0| for every instance, all other ADOs zero; step counter to 0
+// Persistent, software-pipelined variant of stage_mfma for N <= 32 with at most
+// KM modes per bath and 2 K N <= 512: each workgroup walks the tiles
+// t = blockIdx.x, +gridDim.x, ... and, while its four waves run the MFMA
+// commutator and the epilogue of tile t from LDS buffer b, the loads of tile
+// t + gridDim.x are already in flight into registers (its neighbour indices were
+// loaded one tile earlier still).  Vector loads retire in order, so every small
+// load the loop body waits on (tile descriptors, instance fields, the V-is-real
+// flag, the damping) is issued before the prefetch and kept in registers; the
+// only waits inside the loop are then the commit of the prefetched tile and the
+// descriptors of the tile two ahead.  rho and the coupling rows/columns are
+// double-buffered in LDS; V is held as real and imaginary planes of doubles (row
+// stride NP+1: conflict-free for the 8-byte operand reads) and restaged only when
+// the tile's instance changes.
+namespace pipe {
+constexpr int NP = 32, LD = NP + 1, SPT = NP * NP / 256, ENT = 2;
+
+struct Tile {   // a tile's descriptor and its instance's fields, in registers
+    int inst, a0, N, K, Km, d, so, ado_off, al_off, v_off, step_off, nsteps;
+    long long state_off, lk_off;
+};
+
+__device__ __forceinline__ Tile tile_of(const StageArgs& A, int t) {
+    Tile T;
+    T.inst = A.tile_inst[t];
+    T.a0 = A.tile_ado0[t];
+    const InstDev& I = A.inst[T.inst];
+    T.N = I.N;
+    T.K = I.K;
+    T.Km = I.Km;
+    T.d = I.d;
+    T.so = I.so;
+    T.ado_off = I.ado_off;
+    T.al_off = I.al_off;
+    T.v_off = I.v_off;
+    T.step_off = I.step_off;
+    T.nsteps = I.nsteps;
+    T.state_off = I.state_off;
+    T.lk_off = I.lk_off;
+    return T;
+}
+
+template <int KM>
+struct Idx {   // neighbour indices of a tile, loaded one tile ahead of its data
+    int up[ENT][KM], dn[ENT][KM];
+};
+template <int KM>
+struct Dat {   // everything else of a tile
+    double2 x[SPT];
+    double2 y[4], a[4];
+    double2 gu[ENT][KM], gd[ENT][KM];
+    double cx[ENT][KM], cy[ENT][KM];
+    double2 al[ENT][KM];
+};
+
+// entry e of the coupling rows/columns -> (bath j, row/column, element)
+__device__ __forceinline__ void entry_of(int e, int N, int so, int& j, int& rc, int& ab) {
+    j = e / (2 * N);
+    const int rem = e - j * 2 * N;
+    rc = rem / N;
+    const int pos = rem - rc * N;
+    const int s = j + so;
+    ab = rc == 0 ? s * N + pos : pos * N + s;
+}
+
+template <int KM>
+__device__ __forceinline__ void load_idx(const StageArgs& A, const Tile& T, Idx<KM>& X) {
+    const int per = 2 * T.K * T.N;
+#pragma unroll
+    for (int r = 0; r < ENT; ++r) {
+        const int e = threadIdx.x + 256 * r;
+        int j = 0, rc = 0, ab = 0;
+        if (e < per) entry_of(e, T.N, T.so, j, rc, ab);
+#pragma unroll
+        for (int k = 0; k < KM; ++k) {
+            X.up[r][k] = -1;
+            X.dn[r][k] = -1;
+            if (e < per && k < T.Km) {
+                const long long q = T.lk_off + (long long)T.a0 * T.d + j + k * T.K;
+                X.up[r][k] = A.nb_up[q];
+                X.dn[r][k] = A.nb_dn[q];
+            }
+        }
+    }
+}
+
+template <int KM>
+__device__ __forceinline__ void load_dat(const StageArgs& A, const Tile& T, const Idx<KM>& X, Dat<KM>& D) {
+    const int N = T.N, NN = N * N;
+    const long long base = T.state_off + (long long)T.a0 * NN;
+#pragma unroll
+    for (int q = 0; q < SPT; ++q) {
+        const int e = threadIdx.x + q * 256;
+        const int r = e / NP, c = e - r * NP;
+        D.x[q] = (r < N && c < N) ? A.rin[base + r * N + c] : make_double2(0.0, 0.0);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ti = wave >> 1, tj = wave & 1, li = lane & 15, lk = lane >> 4;
+    const int col = tj * 16 + li;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int row = ti * 16 + lk + 4 * r;
+        D.y[r] = make_double2(0.0, 0.0);
+        D.a[r] = make_double2(0.0, 0.0);
+        if (row < N && col < N && A.stage != 0) {
+            const size_t gi = (size_t)(base + row * N + col);
+            if (A.stage < 3) D.y[r] = A.y[gi];
+            D.a[r] = A.acc[gi];
+        }
+    }
+    const int per = 2 * T.K * N;
+#pragma unroll
+    for (int r = 0; r < ENT; ++r) {
+        const int e = threadIdx.x + 256 * r;
+        int j = 0, rc = 0, ab = 0;
+        if (e < per) entry_of(e, N, T.so, j, rc, ab);
+#pragma unroll
+        for (int k = 0; k < KM; ++k) {
+            const int mu = X.up[r][k], md = X.dn[r][k];
+            D.gu[r][k] = mu >= 0 ? A.rin[T.state_off + (long long)mu * NN + ab] : make_double2(0.0, 0.0);
+            D.gd[r][k] = md >= 0 ? A.rin[T.state_off + (long long)md * NN + ab] : make_double2(0.0, 0.0);
+            D.cx[r][k] = 0.0;
+            D.cy[r][k] = 0.0;
+            D.al[r][k] = make_double2(0.0, 0.0);
+            if (e < per && k < T.Km) {
+                const long long q = T.lk_off + (long long)T.a0 * T.d + j + k * T.K;
+                D.cx[r][k] = A.cx[q];
+                D.cy[r][k] = A.cy[q];
+                D.al[r][k] = rc == 0 ? A.alpha[T.al_off + j * T.Km + k] : A.alpha_t[T.al_off + j * T.Km + k];
+            }
+        }
+    }
+}
+}  // namespace pipe
+
+template <int KM>
+__global__ __launch_bounds__(256, 2) void stage_mfma_pipe(StageArgs A, int ntiles, int cpn, double h2, double h,
+                                                          double hb) {
+    using namespace pipe;
+    extern __shared__ double2 smem[];
+    double2* const sR0 = smem;
+    double2* const sR1 = smem + NP * LD;
+    double* const sVr = reinterpret_cast<double*>(smem + 2 * NP * LD);
+    double* const sVi = sVr + NP * LD;
+    double2* const sC0 = reinterpret_cast<double2*>(sVi + NP * LD);
+    double2* const sC1 = sC0 + cpn;
+    const int step = *A.step_ctr + A.step_off;
+    int t = blockIdx.x;
+    if (t >= ntiles) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int ti = wave >> 1, tj = wave & 1;   // one 16x16 output tile per wave
+    const double gs = A.gsign;
+    const int st = A.stage;
+
+    Idx<KM> ix;
+    Dat<KM> dt;
+    Tile cur = tile_of(A, t);
+    load_idx(A, cur, ix);
+    load_dat(A, cur, ix, dt);
+    int tn = t + gridDim.x;
+    Tile nxt = cur;
+    if (tn < ntiles) {
+        nxt = tile_of(A, tn);
+        load_idx(A, nxt, ix);
+    }
+    int vinst = -1;
+    int b = 0;
+    while (true) {
+        const int N = cur.N;
+        const long long base = cur.state_off + (long long)cur.a0 * (N * N);
+        const bool live = step < cur.nsteps;
+        if (cur.inst != vinst) {   // (re)stage V for this instance
+            __syncthreads();
+            const double2* V = A.V + cur.v_off;
+            for (int e = threadIdx.x; e < NP * NP; e += 256) {
+                const int r = e / NP, c = e - r * NP;
+                const double2 v = (r < N && c < N) ? V[r * N + c] : make_double2(0.0, 0.0);
+                sVr[r * LD + c] = v.x;
+                sVi[r * LD + c] = v.y;
+            }
+            vinst = cur.inst;
+        }
+        const bool vreal = A.vreal[cur.step_off + (live ? step : 0)] != 0;
+        const double2 g = A.damp[cur.ado_off + cur.a0];
+        // commit tile `cur` into LDS buffer b
+        double2* const Rb = b ? sR1 : sR0;
+        double2* const Cb = b ? sC1 : sC0;
+#pragma unroll
+        for (int q = 0; q < SPT; ++q) {
+            const int e = threadIdx.x + q * 256;
+            const int r = e / NP, c = e - r * NP;
+            Rb[r * LD + c] = dt.x[q];
+        }
+        {
+            const int per = 2 * cur.K * N;
+#pragma unroll
+            for (int r = 0; r < ENT; ++r) {
+                const int e = threadIdx.x + 256 * r;
+                if (e < per) {
+                    const int rc = (e - (e / (2 * N)) * 2 * N) / N;
+                    const double sg = rc == 0 ? gs : -gs;
+                    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+                    for (int k = 0; k < KM; ++k) {
+                        const double2 wu = make_double2(0.0, sg * dt.cx[r][k]);
+                        const double2 al = dt.al[r][k];
+                        const double y = dt.cy[r][k];
+                        const double2 wd = make_double2(-sg * al.y * y, sg * al.x * y);
+                        acc = cmad(acc, wu, dt.gu[r][k]);
+                        acc = cmad(acc, wd, dt.gd[r][k]);
+                    }
+                    Cb[e] = acc;
+                }
+            }
+        }
+        double2 ycur[4], acur[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            ycur[r] = dt.y[r];
+            acur[r] = dt.a[r];
+        }
+        // descriptor of the tile after next, before the prefetch is issued
+        const int t_after = tn + gridDim.x;
+        Tile aft = nxt;
+        if (t_after < ntiles) aft = tile_of(A, t_after);
+        __syncthreads();
+        if (tn < ntiles) {
+            load_dat(A, nxt, ix, dt);
+            if (t_after < ntiles) load_idx(A, aft, ix);
+        }
+        // MFMA commutator of tile `cur` from LDS buffer b
+        const int ar = ti * 16 + li, bc = tj * 16 + li;
+        dbl4 cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+        const double2* R = Rb;
+        if (vreal) {
+#pragma unroll 4
+            for (int k0 = 0; k0 < NP; k0 += 4) {
+                const int kk = k0 + lk;
+                const double va = sVr[ar * LD + kk], vb = sVr[kk * LD + bc];
+                const double2 rb = R[kk * LD + bc], ra = R[ar * LD + kk];
+                cr = mfma4(va, rb.x, cr);
+                ci = mfma4(va, rb.y, ci);
+                cr = mfma4(ra.x, -vb, cr);
+                ci = mfma4(ra.y, -vb, ci);
+            }
+        } else {
+#pragma unroll 2
+            for (int k0 = 0; k0 < NP; k0 += 4) {
+                const int kk = k0 + lk;
+                const double var = sVr[ar * LD + kk], vai = sVi[ar * LD + kk];
+                const double vbr = sVr[kk * LD + bc], vbi = sVi[kk * LD + bc];
+                const double2 rb = R[kk * LD + bc], ra = R[ar * LD + kk];
+                cr = mfma4(var, rb.x, cr);
+                ci = mfma4(var, rb.y, ci);
+                cr = mfma4(-vai, rb.y, cr);
+                ci = mfma4(vai, rb.x, ci);
+                cr = mfma4(ra.x, -vbr, cr);
+                ci = mfma4(ra.y, -vbr, ci);
+                cr = mfma4(ra.y, vbi, cr);
+                ci = mfma4(ra.x, -vbi, ci);
+            }
+        }
+        if (live) {
+            const int col = tj * 16 + li;
+            const int K = cur.K, so = cur.so;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = ti * 16 + lk + 4 * r;
+                if (row < N && col < N) {
+                    double kr = ci[r], ki = -cr[r];   // -i * C
+                    const double2 x = R[row * LD + col];
+                    kr = fma(g.x, x.x, fma(-g.y, x.y, kr));
+                    ki = fma(g.x, x.y, fma(g.y, x.x, ki));
+                    const int jr = row - so, jc = col - so;
+                    if (jr >= 0 && jr < K) {
+                        const double2 v = Cb[(jr * 2 + 0) * N + col];
+                        kr += v.x;
+                        ki += v.y;
+                    }
+                    if (jc >= 0 && jc < K) {
+                        const double2 v = Cb[(jc * 2 + 1) * N + row];
+                        kr += v.x;
+                        ki += v.y;
+                    }
+                    const size_t gi = (size_t)(base + row * N + col);
+                    if (st == 0) {
+                        A.acc[gi] = make_double2(fma(hb, kr, x.x), fma(hb, ki, x.y));
+                        A.out[gi] = make_double2(fma(h2, kr, x.x), fma(h2, ki, x.y));
+                    } else if (st < 3) {
+                        const double hs = st == 1 ? h2 : h;
+                        A.acc[gi] = make_double2(fma(hb, kr, acur[r].x), fma(hb, ki, acur[r].y));
+                        A.out[gi] = make_double2(fma(hs, kr, ycur[r].x), fma(hs, ki, ycur[r].y));
+                    } else {
+                        A.out[gi] = make_double2(fma(hb, kr, acur[r].x), fma(hb, ki, acur[r].y));
+                    }
+                }
+            }
+        }
+        if (tn >= ntiles) break;
+        cur = nxt;
+        nxt = aft;
+        tn = t_after;
+        b ^= 1;
+    }
+}
+
 __global__ void reset_state(double2* y, long long total, const long long* ado0_off, int n_inst, int* ctr) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long stride = (long long)gridDim.x * blockDim.x;
@@ -337,6 +649,11 @@ int launch_stage(const Group& g, const StageArgs& base, int tile0, hipStream_t s
 #undef EH_CASE
         case -32: hipLaunchKernelGGL((stage_mfma<32, true>), grid, block, g.shmem, s, a); break;
         case -64: hipLaunchKernelGGL((stage_mfma<64, false>), grid, block, g.shmem, s, a); break;
+        case -33:
+            // the RK coefficients of this stage (one time step for the whole group)
+            hipLaunchKernelGGL(stage_mfma_pipe<2>, dim3(std::min(g.tile_count, g.persist)), block, g.shmem, s, a,
+                               g.tile_count, g.maxkn, g.h2, g.h, g.hb[a.stage]);
+            break;
         default: hipLaunchKernelGGL(stage_generic, grid, block, g.shmem, s, a); break;
     }
     HIPCHK(hipGetLastError());
@@ -403,9 +720,13 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
     int64_t state_off = 0, p_off = 0, lk_off = 0;
     int ado_off = 0, v_off = 0, r_off = 0, step_off = 0, al_off = 0;
     // tiles grouped by kernel variant
-    // kernel variants: 1..16 stage_small<N>, 17 = stage_mfma<32>, 18 = stage_mfma<64>, 0 = stage_generic
-    std::vector<std::vector<int>> tiles_by_nt(19);   // each entry: triples (inst, ado0, nado)
-    const bool use_mfma = !(std::getenv("ETOHEOM_NO_MFMA") && std::getenv("ETOHEOM_NO_MFMA")[0] == '1');
+    // kernel variants: 1..16 stage_small<N>, 17 = stage_mfma<32>, 18 = stage_mfma<64>,
+    // 19 = stage_mfma_pipe (N <= 32, K_m <= 2, 2 K N <= 512), 0 = stage_generic
+    std::vector<std::vector<int>> tiles_by_nt(20);   // each entry: triples (inst, ado0, nado)
+    auto env_on = [](const char* v) { const char* e = std::getenv(v); return e && e[0] == '1'; };
+    const bool use_mfma = !env_on("ETOHEOM_NO_MFMA");
+    const bool use_pipe = env_on("ETOHEOM_PIPE");   // opt-in: slower than stage_mfma<32> as compiled
+    double pipe_h = -1.0, pipe_h2 = 0, pipe_hb[4] = {0, 0, 0, 0};
 
     for (int i = 0; i < n; i++) {
         const eh_instance& q = in[i];
@@ -504,9 +825,18 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
         inf.state_off = state_off;
         inf.p_off = p_off;
         b->ado_updates += (int64_t)M * s.nsteps;
+        if (s.h == pipe_h) {
+            pipe_h2 = s.h2;
+            for (int k = 0; k < 4; k++) pipe_hb[k] = s.hb[k];
+        }
         b->max_steps = std::max(b->max_steps, s.nsteps);
         // tiles
-        const int nt = N <= 16 ? N : (!use_mfma ? 0 : (N <= 32 ? 17 : 18));
+        int nt = N <= 16 ? N : (!use_mfma ? 0 : (N <= 32 ? 17 : 18));
+        if (nt == 17 && use_pipe && sys.Km <= 2 && 2 * sys.K * N <= 512) nt = 19;
+        if (nt == 19) {   // one time step per pipelined group: the coefficients are kernel arguments
+            if (pipe_h < 0) pipe_h = s.h;
+            if (s.h != pipe_h) nt = 17;
+        }
         const int G = nt ? (NN >= 512 ? 1 : 512 / NN) : 1;
         for (int a0 = 0; a0 < M; a0 += G) {
             auto& tv = tiles_by_nt[(size_t)nt];
@@ -525,11 +855,15 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
     }
     b->total_ados = ado_off;
     b->total_elems = state_off;
-    for (int nt = 0; nt <= 18; nt++) {
+    for (int nt = 0; nt <= 19; nt++) {
         const auto& tv = tiles_by_nt[(size_t)nt];
         if (tv.empty()) continue;
         Group g;
-        g.nt = nt <= 16 ? nt : (nt == 17 ? -32 : -64);
+        g.nt = nt <= 16 ? nt : (nt == 17 ? -32 : (nt == 18 ? -64 : -33));
+        g.persist = 2 * ctx->n_cu;
+        g.h = pipe_h;
+        g.h2 = pipe_h2;
+        for (int k = 0; k < 4; k++) g.hb[k] = pipe_hb[k];
         g.tile_begin = (int)t_inst.size();
         g.tile_count = (int)(tv.size() / 3);
         int maxnn = 0, maxkn = 0, maxd = 0;
@@ -546,7 +880,9 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
         g.maxnn = maxnn;
         g.maxkn = maxkn;
         g.maxd = maxd;
-        if (g.nt == -32)
+        if (g.nt == -33)
+            g.shmem = (size_t)2 * 32 * 33 * sizeof(double2) + (size_t)2 * 32 * 33 * sizeof(double) + 2 * cpb;
+        else if (g.nt == -32)
             g.shmem = (size_t)2 * 32 * 33 * sizeof(double2) + cpb;
         else if (g.nt == -64)
             g.shmem = (size_t)64 * 65 * sizeof(double2) + cpb;
@@ -554,10 +890,9 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
             g.shmem = nt ? 0 : (size_t)maxnn * sizeof(double2) + cpb;
         if ((int)g.shmem > ctx->lds_per_block) return fail(EH_ESHAPE, "state tile exceeds LDS");
         if (g.shmem > 65536) {
-            hipError_t fe = g.nt == -64 ? hipFuncSetAttribute((const void*)stage_mfma<64, false>,
-                                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)g.shmem)
-                                        : hipFuncSetAttribute((const void*)stage_generic,
-                                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)g.shmem);
+            const void* fn = g.nt == -64 ? (const void*)stage_mfma<64, false>
+                             : (g.nt == -33 ? (const void*)stage_mfma_pipe<2> : (const void*)stage_generic);
+            hipError_t fe = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)g.shmem);
             if (fe != hipSuccess) return fail(EH_ESHAPE, "kernel cannot use the LDS the tile needs");
         }
         im->groups.push_back(g);

@@ -51,10 +51,12 @@ struct DevBuf {
 };
 
 struct Group {
-    int nt;           // 1..16: stage_small<nt>; -32 / -64: stage_mfma; 0: stage_generic
+    int nt;           // 1..16: stage_small<nt>; -32 / -64: stage_mfma; -33: stage_mfma_pipe; 0: stage_generic
     int tile_begin, tile_count;
     size_t shmem;
     int maxnn, maxkn, maxd;   // largest N^2, 2 K N and d of the group's instances
+    int persist;              // grid of the persistent kernels (workgroups)
+    double h, h2, hb[4];      // RK coefficients of the pipelined group (one time step)
 };
 
 struct AdjState;   // adjoint.hip
