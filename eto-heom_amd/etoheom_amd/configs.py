@@ -15,9 +15,10 @@ Workloads (all FP64, dt = 0.5 fs, RK4):
   chain10      10 sites (N=56), nearest-neighbour J=80 cm^-1, d=20, L=3 -> 1771 ADOs, 3000 steps
 
 The Ohmic bath J(w) = (lambda/wc) w exp(-w/wc), lambda=35 cm^-1, wc=100 cm^-1,
-T=77 K, is represented by one damped cosine fitted as etom.fit_ohmic does
-(two ETO modes, +/-w); the fitted numbers are kept here so that building a
-workload needs no quadrature.
+T=77 K, is represented by one damped cosine, the fit
+etom.fit_ohmic(wc_cm=100, T=77, n_osc=1, eta=0.35) (two ETO modes, +/-w; w lies on
+a flat direction of that fit, ~0); the fitted numbers are kept here so that
+building a workload needs no quadrature.
 """
 from __future__ import annotations
 
@@ -30,7 +31,7 @@ from . import MODE_2DES, Param, Problem
 
 HBAR = 5308.0
 
-# etom.fit_ohmic(lam=35, wc=100, T=77, n_osc=1): C(t) ~ a cos(wt) e^{-gt} + i b cos(wt) e^{-gt}
+# etom.fit_ohmic(wc_cm=100, T=77, n_osc=1, eta=0.35): C(t) ~ (a + i b) cos(wt) e^{-gt}, lambda = eta wc / pi
 OHMIC_35_100_77 = {
     "lambda_line": 35.0 / HBAR / np.pi,
     "a": 6.736034787240873e-05,
