@@ -72,6 +72,28 @@ def cpu_baseline(p, rho, seconds=20.0):
                       f"{elapsed:.1f} s, 1 thread"}
 
 
+def per_propagation(ctx, items, k):
+    """The plain per-launch GPU path on k of the workload's propagations (evenly
+    spaced over the list): each one alone, its kernels launched one by one from
+    the host, step after step.  Set-up (hierarchy, tables, uploads) is outside the
+    timed region, as it is for the batched path."""
+    import numpy as np
+    import etoheom_amd as E
+    idx = sorted(set(np.linspace(0, len(items) - 1, k).astype(int).tolist()))
+    bs = [E.Batch(ctx, [{"problem": items[i], "H": items[i].H, "X": items[i].X, "nv": (1, 1, 1)}]) for i in idx]
+    bs[0].run_eager()   # warm-up
+    bs[0].sync()
+    t0 = time.perf_counter()
+    for b in bs:
+        b.run_eager()
+        b.sync()
+    wall = time.perf_counter() - t0
+    for b in bs:
+        b.close()
+    return {"propagations": len(idx), "seconds": wall, "propagations_per_s": len(idx) / wall,
+            "sample": f"{len(idx)} of {len(items)} propagations, evenly spaced, one at a time, per-launch (no graph)"}
+
+
 def traffic_of(workload):
     """HBM bytes per stage launch from the committed rocprofv3 --pmc summary, if any."""
     f = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
@@ -132,6 +154,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--nsteps", type=int, default=0, help="shorten the propagation (counter runs only)")
+    ap.add_argument("--autotune", type=int, default=0, help="autotuner steps per launch candidate (0: off)")
+    ap.add_argument("--per-propagation", type=int, default=0,
+                    help="also time this many of the workload's propagations one at a time on the plain "
+                         "per-launch path (no graph, no batching)")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="also run a plain 2 GiB read + 2 GiB write elementwise copy (counter scale)")
     a = ap.parse_args()
@@ -156,6 +182,7 @@ def main():
     items, desc = workload(a.workload, a.nsteps)
     ctx = E.Context(0)
     batch = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)} for p in items])
+    tuning = batch.autotune(a.autotune).splitlines() if a.autotune else None
     for _ in range(a.warmup):
         batch.run()
         batch.sync()
@@ -178,6 +205,8 @@ def main():
     bytes_per_launch = 64.0 * elems        # 256 N^2 B per ADO-update over 4 stage launches
     achieved = bytes_per_launch / (stage_ms * 1e-3) / 1e9
 
+    per_prop = per_propagation(ctx, items, a.per_propagation) if a.per_propagation else None
+
     traffic, tsrc = traffic_of(a.workload)
     ups = batch.ado_updates * a.steps
     value = ups / wall
@@ -197,12 +226,18 @@ def main():
         "config": {"workload": desc, "propagations_per_step": len(items),
                    "propagations_per_s": len(items) * a.steps / wall,
                    "grid_points_per_s": len(items) * a.steps / wall / 3.0,
-                   "gpu_event_ms_last_run": gpu_ms, "device": ctx.name, "finite": finite},
+                   "gpu_event_ms_last_run": gpu_ms, "device": ctx.name, "finite": finite,
+                   "groups": batch.tuning_report().splitlines()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": tsrc,
                      "kernel": "stage (fused RHS + RK4 stage)", "stage_ms": [float(x) for x in prof[1:5]],
                      "prologue_ms": float(prof[0]), "bytes_per_launch": bytes_per_launch},
     }
+    if tuning:
+        rec["config"]["autotune_steps"] = a.autotune
+    if per_prop:
+        per_prop["batched_over_per_propagation"] = rec["config"]["propagations_per_s"] / per_prop["propagations_per_s"]
+        rec["config"]["per_propagation_path"] = per_prop
     if not a.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(items[0], rho_final)
     print(json.dumps(rec))

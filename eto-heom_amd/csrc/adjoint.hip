@@ -546,13 +546,9 @@ int batch_enable_adjoint(Batch* b, int C) {
         pa.step_off = k;
         hipLaunchKernelGGL(prologue, dim3(b->n_inst), dim3(256), 0, s, pa);
         for (int st = 0; st < 4; st++) {
-            StageArgs x = fw;
-            x.step_off = k;
-            x.stage = st;
-            x.rin = st == 0 ? im->Y.p : (st == 2 ? im->S2.p : im->S1.p);
-            x.out = st == 3 ? im->Y.p : (st == 1 ? im->S2.p : im->S1.p);
+            const StageArgs x = stage_io(fw, im, k, st);
             for (const Group& g : im->groups)
-                if ((rc = launch_stage(g, x, g.tile_begin, s))) {
+                if ((rc = launch_stage(g, x, s))) {
                     hipGraph_t junk;
                     (void)hipStreamEndCapture(s, &junk);
                     return rc;
@@ -616,7 +612,7 @@ int batch_enable_adjoint(Batch* b, int C) {
             x.rin = stg(k, st);
             x.out = st == 3 ? stg(k + 1, 0) : stg(k, st + 1);
             for (const Group& g : im->groups)
-                if ((rc = launch_stage(g, x, g.tile_begin, s))) {
+                if ((rc = launch_stage(g, x, s))) {
                     hipGraph_t junk;
                     (void)hipStreamEndCapture(s, &junk);
                     return rc;

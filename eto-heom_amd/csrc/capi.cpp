@@ -1,4 +1,5 @@
 // capi.cpp -- the extern "C" entry points declared in include/etoheom.h.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -282,6 +283,27 @@ int eh_batch_get_output(eh_batch* bb, int i, double* out, int64_t cap, int64_t* 
 
 int eh_batch_profile(eh_batch* b, int nsteps, double* out5) {
     return b ? batch_profile(b->b, nsteps, out5) : fail(EH_EINVAL, "NULL batch");
+}
+
+int eh_batch_run_eager(eh_batch* b) { return b ? batch_run_eager(b->b) : fail(EH_EINVAL, "NULL batch"); }
+
+int eh_batch_autotune(eh_batch* b, int steps) {
+    return b ? batch_autotune(b->b, steps) : fail(EH_EINVAL, "NULL batch");
+}
+
+int eh_batch_select(eh_batch* b, int group, int cand) {
+    return b ? batch_select(b->b, group, cand) : fail(EH_EINVAL, "NULL batch");
+}
+
+int64_t eh_batch_tuning_report(const eh_batch* b, char* buf, int64_t cap) {
+    if (!b) return fail(EH_EINVAL, "NULL batch");
+    const std::string r = batch_tuning_report(b->b);
+    if (buf && cap > 0) {
+        const size_t n = std::min<size_t>(r.size(), (size_t)cap - 1);
+        std::memcpy(buf, r.data(), n);
+        buf[n] = 0;
+    }
+    return (int64_t)r.size();
 }
 
 int eh_batch_enable_adjoint(eh_batch* b, int checkpoint_every) {

@@ -149,6 +149,7 @@ __device__ __forceinline__ dbl4 mfma4(double a, double b, dbl4 c) {
 
 struct PrologueArgs {
     const InstDev* inst;
+    const int* ilist;        // instance of each workgroup (NULL: blockIdx.x)
     const int* step_ctr;
     int step_off;
     const double2* H;        // per instance N*N
@@ -165,7 +166,7 @@ struct PrologueArgs {
 // V(t_i) = H - e_1 X[0] - e_2 X[1]^T - e_3 X[2]^T  (dynamics.cpp:348-363), and
 // the readouts Tr(rho_0 R_r) before step i (dynamics.cpp:367-381, :246-253)
 static __global__ __launch_bounds__(256) void prologue(PrologueArgs A) {
-    const InstDev& I = A.inst[blockIdx.x];
+    const InstDev& I = A.inst[A.ilist ? A.ilist[blockIdx.x] : (int)blockIdx.x];
     const int step = *A.step_ctr + A.step_off;
     if (step >= I.nsteps) return;
     const int N = I.N, NN = N * N;

@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -40,11 +41,12 @@ namespace eh {
 
 // ---------------------------------------------------------------- device data
 
-// N <= 16: a tile of G = 512/N^2 ADOs per workgroup, two elements per thread.
-template <int NT>
+// N <= 16: a tile of G = GE/N^2 ADOs per workgroup (GE state elements: 256, 512
+// or 1024, the autotuner's choice), GE/256 elements per thread.
+template <int NT, int GE>
 __global__ __launch_bounds__(256) void stage_small(StageArgs A) {
     constexpr int NN = NT * NT;
-    constexpr int G = NN >= 512 ? 1 : 512 / NN;
+    constexpr int G = NN >= GE ? 1 : GE / NN;
     constexpr int EPT = (G * NN + 255) / 256;
     constexpr int CPMAX = G * 2 * NT * NT;   // K <= N
     __shared__ double2 sR[G * NN];
@@ -635,29 +637,45 @@ int ctx_init(int device, eh_ctx* ctx) {
 
 static inline double2 d2(cplx c) { return make_double2(c.real(), c.imag()); }
 
-int launch_stage(const Group& g, const StageArgs& base, int tile0, hipStream_t s) {
+int launch_cfg(const Group& g, const LaunchCfg& c, const StageArgs& base, hipStream_t s) {
     StageArgs a = base;
-    a.tile_inst += tile0;
-    a.tile_ado0 += tile0;
-    a.tile_nado += tile0;
-    const dim3 grid(g.tile_count), block(256);
-    switch (g.nt) {
-#define EH_CASE(n) \
-    case n: hipLaunchKernelGGL(stage_small<n>, grid, block, 0, s, a); break;
+    a.tile_inst += c.tile_begin;
+    a.tile_ado0 += c.tile_begin;
+    a.tile_nado += c.tile_begin;
+    const dim3 grid(c.tile_count), block(256);
+    switch (c.nt) {
+#define EH_CASE(n)                                                                                 \
+    case n:                                                                                        \
+        if (c.ge == 256)                                                                           \
+            hipLaunchKernelGGL((stage_small<n, 256>), grid, block, 0, s, a);                       \
+        else if (c.ge == 1024)                                                                     \
+            hipLaunchKernelGGL((stage_small<n, 1024>), grid, block, 0, s, a);                      \
+        else                                                                                       \
+            hipLaunchKernelGGL((stage_small<n, 512>), grid, block, 0, s, a);                       \
+        break;
         EH_CASE(1) EH_CASE(2) EH_CASE(3) EH_CASE(4) EH_CASE(5) EH_CASE(6) EH_CASE(7) EH_CASE(8)
         EH_CASE(9) EH_CASE(10) EH_CASE(11) EH_CASE(12) EH_CASE(13) EH_CASE(14) EH_CASE(15) EH_CASE(16)
 #undef EH_CASE
-        case -32: hipLaunchKernelGGL((stage_mfma<32, true>), grid, block, g.shmem, s, a); break;
-        case -64: hipLaunchKernelGGL((stage_mfma<64, false>), grid, block, g.shmem, s, a); break;
+        case -32: hipLaunchKernelGGL((stage_mfma<32, true>), grid, block, c.shmem, s, a); break;
+        case -64: hipLaunchKernelGGL((stage_mfma<64, false>), grid, block, c.shmem, s, a); break;
         case -33:
             // the RK coefficients of this stage (one time step for the whole group)
-            hipLaunchKernelGGL(stage_mfma_pipe<2>, dim3(std::min(g.tile_count, g.persist)), block, g.shmem, s, a,
-                               g.tile_count, g.maxkn, g.h2, g.h, g.hb[a.stage]);
+            hipLaunchKernelGGL(stage_mfma_pipe<2>, dim3(std::min(c.tile_count, g.persist)), block, c.shmem, s, a,
+                               c.tile_count, g.maxkn, g.h2, g.h, g.hb[a.stage]);
             break;
-        default: hipLaunchKernelGGL(stage_generic, grid, block, g.shmem, s, a); break;
+        default: hipLaunchKernelGGL(stage_generic, grid, block, c.shmem, s, a); break;
     }
     HIPCHK(hipGetLastError());
     return EH_OK;
+}
+
+StageArgs stage_io(const StageArgs& sa, const BatchImpl* im, int k, int st) {
+    StageArgs a = sa;
+    a.step_off = k;
+    a.stage = st;
+    a.rin = st == 0 ? im->Y.p : (st == 2 ? im->S2.p : im->S1.p);
+    a.out = st == 3 ? im->Y.p : (st == 1 ? im->S2.p : im->S1.p);
+    return a;
 }
 
 StageArgs stage_args(const BatchImpl* im) {
@@ -700,6 +718,49 @@ PrologueArgs prologue_args(const BatchImpl* im) {
     return pa;
 }
 
+// The forward graph: `chunk` steps, each group on its own branch (instances of
+// different groups are independent, so their prologue + stage chains overlap),
+// joined before the step-counter tick.
+static int capture_forward(Batch* b) {
+    BatchImpl* im = b->impl;
+    if (im->exec) (void)hipGraphExecDestroy(im->exec);
+    if (im->graph) (void)hipGraphDestroy(im->graph);
+    im->exec = nullptr;
+    im->graph = nullptr;
+    const StageArgs sa = stage_args(im);
+    const PrologueArgs pa = prologue_args(im);
+    HIPCHK(hipStreamBeginCapture(im->stream, hipStreamCaptureModeThreadLocal));
+    int rc = EH_OK;
+    hipError_t e = hipEventRecord(im->fork_ev, im->stream);
+    for (size_t gi = 0; gi < im->groups.size() && rc == EH_OK && e == hipSuccess; gi++) {
+        const Group& g = im->groups[gi];
+        hipStream_t s = gi == 0 ? im->stream : im->side[gi - 1];
+        if (gi) e = hipStreamWaitEvent(s, im->fork_ev, 0);
+        PrologueArgs pg = pa;
+        pg.ilist = im->ginst.p + g.inst_begin;
+        for (int k = 0; k < b->chunk && rc == EH_OK && e == hipSuccess; k++) {
+            pg.step_off = k;
+            hipLaunchKernelGGL(prologue, dim3(g.inst_count), dim3(256), 0, s, pg);
+            for (int st = 0; st < 4 && rc == EH_OK; st++) rc = launch_stage(g, stage_io(sa, im, k, st), s);
+        }
+        if (gi && e == hipSuccess) {
+            e = hipEventRecord(im->join_ev[gi - 1], s);
+            if (e == hipSuccess) e = hipStreamWaitEvent(im->stream, im->join_ev[gi - 1], 0);
+        }
+    }
+    hipLaunchKernelGGL(tick, dim3(1), dim3(1), 0, im->stream, im->step_ctr.p, b->chunk);
+    hipGraph_t gr = nullptr;
+    const hipError_t ee = hipStreamEndCapture(im->stream, &gr);
+    if (rc || e != hipSuccess || ee != hipSuccess) {
+        if (ee == hipSuccess && gr) (void)hipGraphDestroy(gr);
+        if (rc) return rc;
+        return fail(EH_EHIP, std::string("forward graph capture: ") + hipGetErrorString(e != hipSuccess ? e : ee));
+    }
+    im->graph = gr;
+    HIPCHK(hipGraphInstantiate(&im->exec, im->graph, nullptr, nullptr, 0));
+    return EH_OK;
+}
+
 int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
     if (!ctx || !in || n <= 0 || !out) return fail(EH_EINVAL, "batch_create: bad arguments");
     HIPCHK(hipSetDevice(ctx->device));
@@ -719,14 +780,11 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
     std::vector<long long> hado0;
     int64_t state_off = 0, p_off = 0, lk_off = 0;
     int ado_off = 0, v_off = 0, r_off = 0, step_off = 0, al_off = 0;
-    // tiles grouped by kernel variant
-    // kernel variants: 1..16 stage_small<N>, 17 = stage_mfma<32>, 18 = stage_mfma<64>,
-    // 19 = stage_mfma_pipe (N <= 32, K_m <= 2, 2 K N <= 512), 0 = stage_generic
-    std::vector<std::vector<int>> tiles_by_nt(20);   // each entry: triples (inst, ado0, nado)
+    // instances by N class: 1..16 (each N its own class), 17 (16 < N <= 32), 18 (32 < N <= 64)
+    std::vector<std::vector<int>> by_class(19);
     auto env_on = [](const char* v) { const char* e = std::getenv(v); return e && e[0] == '1'; };
     const bool use_mfma = !env_on("ETOHEOM_NO_MFMA");
-    const bool use_pipe = env_on("ETOHEOM_PIPE");   // opt-in: slower than stage_mfma<32> as compiled
-    double pipe_h = -1.0, pipe_h2 = 0, pipe_hb[4] = {0, 0, 0, 0};
+    const bool use_pipe = env_on("ETOHEOM_PIPE");   // opt-in default: slower than stage_mfma<32> as compiled
 
     for (int i = 0; i < n; i++) {
         const eh_instance& q = in[i];
@@ -825,25 +883,8 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
         inf.state_off = state_off;
         inf.p_off = p_off;
         b->ado_updates += (int64_t)M * s.nsteps;
-        if (s.h == pipe_h) {
-            pipe_h2 = s.h2;
-            for (int k = 0; k < 4; k++) pipe_hb[k] = s.hb[k];
-        }
         b->max_steps = std::max(b->max_steps, s.nsteps);
-        // tiles
-        int nt = N <= 16 ? N : (!use_mfma ? 0 : (N <= 32 ? 17 : 18));
-        if (nt == 17 && use_pipe && sys.Km <= 2 && 2 * sys.K * N <= 512) nt = 19;
-        if (nt == 19) {   // one time step per pipelined group: the coefficients are kernel arguments
-            if (pipe_h < 0) pipe_h = s.h;
-            if (s.h != pipe_h) nt = 17;
-        }
-        const int G = nt ? (NN >= 512 ? 1 : 512 / NN) : 1;
-        for (int a0 = 0; a0 < M; a0 += G) {
-            auto& tv = tiles_by_nt[(size_t)nt];
-            tv.push_back(i);
-            tv.push_back(a0);
-            tv.push_back(std::min(G, M - a0));
-        }
+        by_class[(size_t)(N <= 16 ? N : (N <= 32 ? 17 : 18))].push_back(i);
         state_off += (int64_t)M * NN;
         p_off += (int64_t)nread * s.npoints;
         ado_off += M;
@@ -855,46 +896,95 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
     }
     b->total_ados = ado_off;
     b->total_elems = state_off;
-    for (int nt = 0; nt <= 19; nt++) {
-        const auto& tv = tiles_by_nt[(size_t)nt];
-        if (tv.empty()) continue;
+    // Groups and their launch candidates.  Tiles are (instance, first ADO, ADO count)
+    // triples; a group's candidates with equal ADOs per tile share one tile range.
+    std::vector<int> hginst;
+    for (int cls = 1; cls <= 18; cls++) {
+        const auto& members = by_class[(size_t)cls];
+        if (members.empty()) continue;
         Group g;
-        g.nt = nt <= 16 ? nt : (nt == 17 ? -32 : (nt == 18 ? -64 : -33));
+        g.inst_begin = (int)hginst.size();
+        g.inst_count = (int)members.size();
+        hginst.insert(hginst.end(), members.begin(), members.end());
         g.persist = 2 * ctx->n_cu;
-        g.h = pipe_h;
-        g.h2 = pipe_h2;
-        for (int k = 0; k < 4; k++) g.hb[k] = pipe_hb[k];
-        g.tile_begin = (int)t_inst.size();
-        g.tile_count = (int)(tv.size() / 3);
         int maxnn = 0, maxkn = 0, maxd = 0;
-        for (size_t k = 0; k < tv.size(); k += 3) {
-            t_inst.push_back(tv[k]);
-            t_ado0.push_back(tv[k + 1]);
-            t_nado.push_back(tv[k + 2]);
-            const InstDev& di = hinst[(size_t)tv[k]];
+        bool pipe_ok = cls == 17;
+        const InstDev& f0 = hinst[(size_t)members[0]];
+        for (int i : members) {
+            const InstDev& di = hinst[(size_t)i];
             maxnn = std::max(maxnn, di.N * di.N);
             maxkn = std::max(maxkn, 2 * di.K * di.N);
             maxd = std::max(maxd, di.d);
+            // the pipelined kernel takes the RK coefficients as arguments: one time step per group
+            if (di.Km > 2 || 2 * di.K * di.N > 512 || di.h != f0.h) pipe_ok = false;
         }
-        const size_t cpb = (size_t)maxkn * sizeof(double2);   // coupling rows/columns
         g.maxnn = maxnn;
         g.maxkn = maxkn;
         g.maxd = maxd;
-        if (g.nt == -33)
-            g.shmem = (size_t)2 * 32 * 33 * sizeof(double2) + (size_t)2 * 32 * 33 * sizeof(double) + 2 * cpb;
-        else if (g.nt == -32)
-            g.shmem = (size_t)2 * 32 * 33 * sizeof(double2) + cpb;
-        else if (g.nt == -64)
-            g.shmem = (size_t)64 * 65 * sizeof(double2) + cpb;
-        else
-            g.shmem = nt ? 0 : (size_t)maxnn * sizeof(double2) + cpb;
-        if ((int)g.shmem > ctx->lds_per_block) return fail(EH_ESHAPE, "state tile exceeds LDS");
-        if (g.shmem > 65536) {
-            const void* fn = g.nt == -64 ? (const void*)stage_mfma<64, false>
-                             : (g.nt == -33 ? (const void*)stage_mfma_pipe<2> : (const void*)stage_generic);
-            hipError_t fe = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)g.shmem);
-            if (fe != hipSuccess) return fail(EH_ESHAPE, "kernel cannot use the LDS the tile needs");
+        g.h = f0.h;
+        g.h2 = f0.h2;
+        for (int k = 0; k < 4; k++) g.hb[k] = f0.hb[k];
+        std::vector<std::pair<int, std::pair<int, int>>> ranges;   // ADOs per tile -> (begin, count)
+        auto tiles = [&](int G) {
+            for (const auto& r : ranges)
+                if (r.first == G) return r.second;
+            const int begin = (int)t_inst.size();
+            for (int i : members) {
+                const int M = hinst[(size_t)i].M;
+                for (int a0 = 0; a0 < M; a0 += G) {
+                    t_inst.push_back(i);
+                    t_ado0.push_back(a0);
+                    t_nado.push_back(std::min(G, M - a0));
+                }
+            }
+            const std::pair<int, int> r(begin, (int)t_inst.size() - begin);
+            ranges.push_back({G, r});
+            return r;
+        };
+        const size_t cpb = (size_t)maxkn * sizeof(double2);   // coupling rows/columns
+        auto add = [&](int nt, int ge) {
+            LaunchCfg c;
+            c.nt = nt;
+            c.ge = ge;
+            const int G = nt > 0 ? (maxnn >= ge ? 1 : ge / maxnn) : 1;
+            const std::pair<int, int> r = tiles(G);
+            c.tile_begin = r.first;
+            c.tile_count = r.second;
+            if (nt == -33)
+                c.shmem = (size_t)2 * 32 * 33 * sizeof(double2) + (size_t)2 * 32 * 33 * sizeof(double) + 2 * cpb;
+            else if (nt == -32)
+                c.shmem = (size_t)2 * 32 * 33 * sizeof(double2) + cpb;
+            else if (nt == -64)
+                c.shmem = (size_t)64 * 65 * sizeof(double2) + cpb;
+            else
+                c.shmem = nt > 0 ? 0 : (size_t)maxnn * sizeof(double2) + cpb;
+            if ((int)c.shmem > ctx->lds_per_block) return;   // not a candidate on this device
+            if (c.shmem > 65536) {
+                const void* fn = nt == -64 ? (const void*)stage_mfma<64, false>
+                                 : (nt == -33 ? (const void*)stage_mfma_pipe<2> : (const void*)stage_generic);
+                if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.shmem) != hipSuccess)
+                    return;
+            }
+            g.cand.push_back(c);
+        };
+        if (cls <= 16) {
+            add(cls, 512);   // the base configuration: the adjoint's small kernels tile as it does
+            add(cls, 256);
+            add(cls, 1024);
+            add(0, 0);
+        } else {
+            if (use_mfma) add(cls == 17 ? -32 : -64, 0);
+            add(0, 0);
+            if (pipe_ok) add(-33, 0);
         }
+        if (g.cand.empty() || (cls <= 16 && g.cand[0].nt != cls))
+            return fail(EH_ESHAPE, "state tile exceeds LDS");
+        for (size_t c = 0; c < g.cand.size(); c++)
+            if (use_pipe && g.cand[c].nt == -33) g.sel = (int)c;
+        g.nt = g.cand[0].nt;
+        g.tile_begin = g.cand[0].tile_begin;
+        g.tile_count = g.cand[0].tile_count;
+        g.shmem = g.cand[0].shmem;
         im->groups.push_back(g);
     }
 
@@ -905,7 +995,8 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
         (rc = im->cx.upload(hcx)) || (rc = im->cy.upload(hcy)) || (rc = im->pidx.upload(hpidx)) ||
         (rc = im->vreal.upload(hvreal)) || (rc = im->H.upload(hH)) || (rc = im->Xp.upload(hXp)) ||
         (rc = im->field.upload(hfield)) || (rc = im->R.upload(hR)) || (rc = im->alpha.upload(hal)) ||
-        (rc = im->alpha_t.upload(halt)) || (rc = im->damp.upload(hdamp)) || (rc = im->ado0_off.upload(hado0)))
+        (rc = im->alpha_t.upload(halt)) || (rc = im->damp.upload(hdamp)) || (rc = im->ado0_off.upload(hado0)) ||
+        (rc = im->ginst.upload(hginst)))
         return rc;
     if ((rc = im->step_ctr.alloc(1)) || (rc = im->V.alloc(hH.size())) || (rc = im->P.alloc((size_t)p_off)) ||
         (rc = im->Y.alloc((size_t)state_off)) || (rc = im->ACC.alloc((size_t)state_off)) ||
@@ -924,36 +1015,25 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
     HIPCHK(hipEventCreate(&im->ev0));
     HIPCHK(hipEventCreate(&im->ev1));
 
-    // capture `chunk` steps into one graph
-    const int per_step = 1 + 4 * (int)im->groups.size();
+    HIPCHK(hipEventCreateWithFlags(&im->fork_ev, hipEventDisableTiming));
+    for (size_t gi = 1; gi < im->groups.size(); gi++) {
+        hipStream_t q = nullptr;
+        HIPCHK(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
+        im->side.push_back(q);
+        hipEvent_t e = nullptr;
+        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        im->join_ev.push_back(e);
+    }
+    // graph of `chunk` steps
+    const int per_step = 5 * (int)im->groups.size();
     b->chunk = std::max(1, std::min(b->max_steps, 256 / per_step));
     im->nchunks = (b->max_steps + b->chunk - 1) / b->chunk;
-    const StageArgs sa = stage_args(im.get());
-    PrologueArgs pa = prologue_args(im.get());
-    HIPCHK(hipStreamBeginCapture(im->stream, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < b->chunk; k++) {
-        pa.step_off = k;
-        hipLaunchKernelGGL(prologue, dim3(n), dim3(256), 0, im->stream, pa);
-        for (int st = 0; st < 4; st++) {
-            StageArgs a = sa;
-            a.step_off = k;
-            a.stage = st;
-            a.rin = st == 0 ? im->Y.p : (st == 2 ? im->S2.p : im->S1.p);
-            a.out = st == 3 ? im->Y.p : (st == 1 ? im->S2.p : im->S1.p);
-            for (const Group& g : im->groups) {
-                rc = launch_stage(g, a, g.tile_begin, im->stream);
-                if (rc) {
-                    hipGraph_t junk;
-                    (void)hipStreamEndCapture(im->stream, &junk);
-                    return rc;
-                }
-            }
-        }
+    b->impl = im.get();
+    if ((rc = capture_forward(b.get()))) {
+        b->impl = nullptr;
+        return rc;
     }
-    hipLaunchKernelGGL(tick, dim3(1), dim3(1), 0, im->stream, im->step_ctr.p, b->chunk);
-    HIPCHK(hipStreamEndCapture(im->stream, &im->graph));
-    HIPCHK(hipGraphInstantiate(&im->exec, im->graph, nullptr, nullptr, 0));
-    b->impl = im.release();
+    im.release();
     *out = b.release();
     return EH_OK;
 }
@@ -1037,13 +1117,8 @@ int batch_profile(Batch* b, int nsteps, double* out) {
         hipLaunchKernelGGL(prologue, dim3(b->n_inst), dim3(256), 0, im->stream, pa);
         (void)hipEventRecord(e[1], im->stream);
         for (int st = 0; st < 4 && rc == EH_OK; st++) {
-            StageArgs a = sa;
-            a.step_off = 0;
-            a.stage = st;
-            a.rin = st == 0 ? im->Y.p : (st == 2 ? im->S2.p : im->S1.p);
-            a.out = st == 3 ? im->Y.p : (st == 1 ? im->S2.p : im->S1.p);
             for (const Group& g : im->groups)
-                if ((rc = launch_stage(g, a, g.tile_begin, im->stream))) break;
+                if ((rc = launch_stage(g, stage_io(sa, im, 0, st), im->stream))) break;
             (void)hipEventRecord(e[2 + st], im->stream);
         }
         hipLaunchKernelGGL(tick, dim3(1), dim3(1), 0, im->stream, im->step_ctr.p, 1);
@@ -1065,6 +1140,124 @@ int batch_profile(Batch* b, int nsteps, double* out) {
     HIPCHK(se);
     b->synced = false;
     return EH_OK;
+}
+
+// The plain per-launch path: the same kernels launched one by one from the host,
+// step after step (prologue, four stages per group, counter tick), no graph.
+int batch_run_eager(Batch* b) {
+    if (!b || !b->impl) return fail(EH_EINVAL, "batch_run_eager: null batch");
+    BatchImpl* im = b->impl;
+    if (im->adj) return fail(EH_EINVAL, "batch_run_eager: the adjoint needs the checkpointed run (eh_batch_run)");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    const long long total = b->total_elems;
+    const int blocks = (int)std::min<long long>(std::max<long long>(1, (total + 255) / 256), 65536);
+    hipLaunchKernelGGL(reset_state, dim3(blocks), dim3(256), 0, im->stream, im->Y.p, total, im->ado0_off.p,
+                       b->n_inst, im->step_ctr.p);
+    hipLaunchKernelGGL(set_ground, dim3((b->n_inst + 255) / 256), dim3(256), 0, im->stream, im->Y.p,
+                       im->ado0_off.p, b->n_inst);
+    HIPCHK(hipEventRecord(im->ev0, im->stream));
+    const StageArgs sa = stage_args(im);
+    const PrologueArgs pa = prologue_args(im);
+    for (int k = 0; k < b->max_steps; k++) {
+        hipLaunchKernelGGL(prologue, dim3(b->n_inst), dim3(256), 0, im->stream, pa);
+        for (int st = 0; st < 4; st++)
+            for (const Group& g : im->groups) {
+                const int rc = launch_stage(g, stage_io(sa, im, 0, st), im->stream);
+                if (rc) return rc;
+            }
+        hipLaunchKernelGGL(tick, dim3(1), dim3(1), 0, im->stream, im->step_ctr.p, 1);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(im->ev1, im->stream));
+    im->ran = true;
+    b->synced = false;
+    return EH_OK;
+}
+
+// Autotuner: for every group, time `steps` RK4 steps (four stage launches each) of
+// every launch candidate -- tile sizes of the small-N kernel, the MFMA, pipelined and
+// generic kernels -- and keep the fastest (a candidate replaces the base one only
+// when it is at least 3% faster), then re-capture the forward graph.  The state
+// the timing runs leave is reset by the next run.
+int batch_autotune(Batch* b, int steps) {
+    if (!b || !b->impl || steps <= 0) return fail(EH_EINVAL, "batch_autotune: bad arguments");
+    BatchImpl* im = b->impl;
+    if (im->adj) return fail(EH_EINVAL, "batch_autotune: tune before enabling the adjoint");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    const long long total = b->total_elems;
+    const int blocks = (int)std::min<long long>(std::max<long long>(1, (total + 255) / 256), 65536);
+    hipLaunchKernelGGL(reset_state, dim3(blocks), dim3(256), 0, im->stream, im->Y.p, total, im->ado0_off.p,
+                       b->n_inst, im->step_ctr.p);
+    hipLaunchKernelGGL(set_ground, dim3((b->n_inst + 255) / 256), dim3(256), 0, im->stream, im->Y.p,
+                       im->ado0_off.p, b->n_inst);
+    const StageArgs sa = stage_args(im);
+    const PrologueArgs pa = prologue_args(im);
+    hipLaunchKernelGGL(prologue, dim3(b->n_inst), dim3(256), 0, im->stream, pa);   // V of step 0
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int rc = EH_OK;
+    for (Group& g : im->groups) {
+        g.cand_ms.assign(g.cand.size(), 0.0);
+        if (g.cand.size() < 2) continue;
+        for (size_t c = 0; c < g.cand.size() && rc == EH_OK; c++) {
+            for (int st = 0; st < 4 && rc == EH_OK; st++)   // warm-up step
+                rc = launch_cfg(g, g.cand[c], stage_io(sa, im, 0, st), im->stream);
+            if (hipEventRecord(e0, im->stream) != hipSuccess) rc = fail(EH_EHIP, "autotune: event record");
+            for (int k = 0; k < steps && rc == EH_OK; k++)
+                for (int st = 0; st < 4 && rc == EH_OK; st++)
+                    rc = launch_cfg(g, g.cand[c], stage_io(sa, im, 0, st), im->stream);
+            if (rc == EH_OK && hipEventRecord(e1, im->stream) != hipSuccess) rc = fail(EH_EHIP, "autotune: event record");
+            if (rc == EH_OK && hipEventSynchronize(e1) != hipSuccess) rc = fail(EH_EHIP, "autotune: synchronize");
+            float ms = 0.0f;
+            if (rc == EH_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = fail(EH_EHIP, "autotune: timing");
+            g.cand_ms[c] = (double)ms / steps;
+        }
+        if (rc) break;
+        size_t best = 0;
+        for (size_t c = 1; c < g.cand.size(); c++)
+            if (g.cand_ms[c] < 0.97 * g.cand_ms[best]) best = c;
+        g.sel = (int)best;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    return capture_forward(b);
+}
+
+// Manual choice of group g's launch candidate c (the tuning report lists both), then re-capture.
+int batch_select(Batch* b, int g, int c) {
+    if (!b || !b->impl) return fail(EH_EINVAL, "batch_select: null batch");
+    BatchImpl* im = b->impl;
+    if (im->adj) return fail(EH_EINVAL, "batch_select: select before enabling the adjoint");
+    if (g < 0 || g >= (int)im->groups.size()) return fail(EH_ERANGE, "batch_select: no such group");
+    if (c < 0 || c >= (int)im->groups[(size_t)g].cand.size()) return fail(EH_ERANGE, "batch_select: no such candidate");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    HIPCHK(hipStreamSynchronize(im->stream));
+    im->groups[(size_t)g].sel = c;
+    return capture_forward(b);
+}
+
+// One line per group and candidate: "group <i> N<=<n> <kernel> tile=<ge> ms=<t> [*]"
+std::string batch_tuning_report(const Batch* b) {
+    std::string out;
+    if (!b || !b->impl) return out;
+    const BatchImpl* im = b->impl;
+    for (size_t gi = 0; gi < im->groups.size(); gi++) {
+        const Group& g = im->groups[gi];
+        for (size_t c = 0; c < g.cand.size(); c++) {
+            const LaunchCfg& x = g.cand[c];
+            const char* kn = x.nt > 0 ? "stage_small" : (x.nt == -32 ? "stage_mfma32" : (x.nt == -64 ? "stage_mfma64"
+                             : (x.nt == -33 ? "stage_mfma_pipe" : "stage_generic")));
+            char line[256];
+            std::snprintf(line, sizeof(line),
+                          "group %zu cand %zu maxN^2=%d instances=%d %s tile=%d tiles=%d ms_per_step=%.6f%s\n", gi, c,
+                          g.maxnn, g.inst_count, kn, x.nt > 0 ? x.ge : 0, x.tile_count,
+                          g.cand_ms.empty() ? -1.0 : g.cand_ms[c], (int)c == g.sel ? " *" : "");
+            out += line;
+        }
+    }
+    return out;
 }
 
 int batch_get_rho(Batch* b, int i, double* rho_out) {

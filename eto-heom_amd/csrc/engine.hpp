@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "host.hpp"
@@ -41,6 +42,10 @@ int batch_sync(Batch* b);
 int batch_elapsed_ms(Batch* b, double* ms);
 int batch_get_rho(Batch* b, int i, double* rho_out);
 int batch_profile(Batch* b, int nsteps, double* out5);
+int batch_run_eager(Batch* b);
+int batch_autotune(Batch* b, int steps);
+int batch_select(Batch* b, int group, int cand);
+std::string batch_tuning_report(const Batch* b);
 int batch_enable_adjoint(Batch* b, int checkpoint_every);
 int batch_backward(Batch* b, const double* dP);
 int batch_get_grad(Batch* b, int i, double* dH, double* dX4, double* dalpha, double* dalpha_t, double* dgamma);

@@ -50,13 +50,26 @@ struct DevBuf {
     }
 };
 
+// One way of launching a group's stage: kernel variant and tiling.
+struct LaunchCfg {
+    int nt;                   // 1..16: stage_small<nt>; -32 / -64: stage_mfma; -33: stage_mfma_pipe; 0: stage_generic
+    int ge;                   // stage_small: state elements per tile (256, 512 or 1024)
+    int tile_begin, tile_count;
+    size_t shmem;
+};
+
+// The instances of one N class (N <= 16 each its own class; 16 < N <= 32; 32 < N <= 64).
 struct Group {
-    int nt;           // 1..16: stage_small<nt>; -32 / -64: stage_mfma; -33: stage_mfma_pipe; 0: stage_generic
+    int nt;                   // base configuration (cand[0]); the adjoint kernels use its tiling
     int tile_begin, tile_count;
     size_t shmem;
     int maxnn, maxkn, maxd;   // largest N^2, 2 K N and d of the group's instances
     int persist;              // grid of the persistent kernels (workgroups)
-    double h, h2, hb[4];      // RK coefficients of the pipelined group (one time step)
+    double h, h2, hb[4];      // RK coefficients of the pipelined variant (one time step)
+    int inst_begin, inst_count;   // the group's instances, a range of BatchImpl::ginst
+    std::vector<LaunchCfg> cand;  // forward launch candidates, cand[0] = the base configuration
+    std::vector<double> cand_ms;  // autotuner timings per candidate (ms per RK4 step), empty if not tuned
+    int sel = 0;                  // the candidate the forward graph launches
 };
 
 struct AdjState;   // adjoint.hip
@@ -73,7 +86,11 @@ struct BatchImpl {
     DevBuf<double> cx, cy;
     DevBuf<double2> V, H, Xp, field, R, P, alpha, alpha_t, damp, Y, ACC, S1, S2;
     DevBuf<long long> ado0_off;
+    DevBuf<int> ginst;                   // instance indices, grouped
     std::vector<Group> groups;
+    std::vector<hipStream_t> side;       // one capture branch per group beyond the first
+    hipEvent_t fork_ev = nullptr;
+    std::vector<hipEvent_t> join_ev;
     int nchunks = 0;
     bool ran = false;
     // host copies the adjoint needs to build its tables
@@ -90,11 +107,20 @@ struct BatchImpl {
         if (graph) (void)hipGraphDestroy(graph);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
+        for (hipEvent_t e : join_ev) (void)hipEventDestroy(e);
+        for (hipStream_t q : side) (void)hipStreamDestroy(q);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
-int launch_stage(const Group& g, const StageArgs& base, int tile0, hipStream_t s);
+int launch_cfg(const Group& g, const LaunchCfg& c, const StageArgs& base, hipStream_t s);
+// the group's selected candidate
+inline int launch_stage(const Group& g, const StageArgs& base, hipStream_t s) {
+    return launch_cfg(g, g.cand[(size_t)g.sel], base, s);
+}
+// stage st of step k: input and output buffers of the RK4 scheme
+StageArgs stage_io(const StageArgs& sa, const BatchImpl* im, int k, int st);
 int adj_forward(Batch* b);
 StageArgs stage_args(const BatchImpl* im);
 PrologueArgs prologue_args(const BatchImpl* im);

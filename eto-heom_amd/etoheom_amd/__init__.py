@@ -42,7 +42,8 @@ EXPORTS = [
     "eh_2des_key_run", "eh_dynamics_key_run", "eh_batch_create", "eh_batch_destroy", "eh_batch_run",
     "eh_batch_sync", "eh_batch_elapsed_ms", "eh_batch_get_output", "eh_batch_get_rho", "eh_batch_dims",
     "eh_batch_ado_updates", "eh_batch_profile", "eh_batch_enable_adjoint", "eh_batch_backward",
-    "eh_batch_get_grad",
+    "eh_batch_get_grad", "eh_batch_run_eager", "eh_batch_autotune", "eh_batch_tuning_report",
+    "eh_batch_select",
 ]
 
 
@@ -116,6 +117,10 @@ def lib():
             "eh_batch_enable_adjoint": (C.c_int, [VP, C.c_int]),
             "eh_batch_backward": (C.c_int, [VP, VP]),
             "eh_batch_get_grad": (C.c_int, [VP, C.c_int, VP, VP, VP, VP, VP]),
+            "eh_batch_run_eager": (C.c_int, [VP]),
+            "eh_batch_autotune": (C.c_int, [VP, C.c_int]),
+            "eh_batch_select": (C.c_int, [VP, C.c_int, C.c_int]),
+            "eh_batch_tuning_report": (I64, [VP, C.c_char_p, I64]),
         }
         for name, (res, args) in sig.items():
             f = getattr(l, name)
@@ -447,6 +452,33 @@ class Batch:
         out = np.zeros(5)
         _check(lib().eh_batch_profile(self._h, nsteps, out.ctypes.data))
         return out
+
+    def run_eager(self):
+        """The same propagation with every kernel launched one by one from the host (no graph)."""
+        _check(lib().eh_batch_run_eager(self._h))
+
+    def autotune(self, steps: int = 5) -> str:
+        """Pick the fastest launch candidate per N class; returns the tuning report."""
+        _check(lib().eh_batch_autotune(self._h, steps))
+        return self.tuning_report()
+
+    def select(self, group: int, cand: int):
+        """Launch candidate `cand` for group `group` (as numbered in tuning_report)."""
+        _check(lib().eh_batch_select(self._h, group, cand))
+
+    def candidates(self):
+        """[(group, cand, kernel, tile)] parsed from the tuning report."""
+        out = []
+        for line in self.tuning_report().splitlines():
+            f = line.split()
+            out.append((int(f[1]), int(f[3]), f[6], int(f[7].split("=")[1])))
+        return out
+
+    def tuning_report(self) -> str:
+        n = lib().eh_batch_tuning_report(self._h, None, 0)
+        buf = C.create_string_buffer(int(n) + 1)
+        lib().eh_batch_tuning_report(self._h, buf, n + 1)
+        return buf.value.decode()
 
     def rho(self, i: int):
         p = self.items[i]["problem"]

@@ -156,6 +156,21 @@ int eh_batch_dims(const eh_batch* b, int64_t* out5);
 /* per-kernel timing pass: nsteps steps launched without the graph, HIP events
  * around each launch on the batch stream; out5 = mean ms of {prologue, stage 0..3} */
 int eh_batch_profile(eh_batch* b, int nsteps, double* out5);
+/* the plain per-launch path: same kernels and results as eh_batch_run, launched one
+ * by one from the host step after step, no graph (the reference GPU path's
+ * per-propagation launch pattern, etoheom-gpu/src/HEOM_dynamics.cu:494-560); for
+ * comparison with the graph-replayed batch; asynchronous like eh_batch_run */
+int eh_batch_run_eager(eh_batch* b);
+/* autotuner: time `steps` RK4 steps of every launch candidate of every N class
+ * (small-N tile sizes of 256/512/1024 state elements, MFMA / pipelined / generic
+ * kernels), keep the fastest and re-capture the graph; call before
+ * eh_batch_enable_adjoint.  eh_batch_tuning_report writes one line per candidate
+ * ("... ms_per_step=<t>", '*' marks the chosen one) into buf (NUL-terminated,
+ * truncated to cap) and returns the full length. */
+int eh_batch_autotune(eh_batch* b, int steps);
+/* manual choice: group g launches its candidate c (numbered as in the report) */
+int eh_batch_select(eh_batch* b, int group, int cand);
+int64_t eh_batch_tuning_report(const eh_batch* b, char* buf, int64_t cap);
 /* ---- gradients (discrete adjoint of the RK4 propagation) -------------------
  * No reference counterpart (the reference has no derivatives); this is the
  * "adjoint gradient" of the propagation path.  After enable, eh_batch_run stores

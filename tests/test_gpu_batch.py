@@ -1,0 +1,105 @@
+"""GPU batch engine: the per-group graph, the plain per-launch path, the launch
+candidates and the autotuner, each against the CPU oracle (<= 1e-10 normwise
+relative, FP64) on one batch that holds four N classes (N = 4, 7, 11, 29)."""
+import numpy as np
+import pytest
+
+import etoheom_amd as E
+import oracle as O
+from etoheom_amd import configs
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return E.Context(0)
+
+
+def oproblem(p):
+    return O.Problem(H=p.H, X=p.X, alpha=p.alpha, alpha_t=p.alpha_t, gamma=p.gamma, L=p.L,
+                     site_offset=p.site_offset, pulses=p.pulses, t_start=p.t_start, t_end=p.t_end,
+                     step_size=p.step_size, print_step=p.print_step)
+
+
+def rel(a, b):
+    b = np.asarray(b)
+    assert b.size and np.max(np.abs(b)) > 0, "reference is identically zero: the check would be vacuous"
+    return np.max(np.abs(np.asarray(a) - b)) / np.max(np.abs(b))
+
+
+def mixed_items():
+    items = [configs.aggregate(2, 3, nsteps=120, taus=(10.0, 20.0, 30.0)),      # N = 4
+             configs.aggregate(3, 3, nsteps=100, taus=(10.0, 25.0, 35.0)),      # N = 7
+             configs.aggregate(4, 2, nsteps=90, taus=(20.0, 10.0, 30.0)),       # N = 11
+             configs.heptamer(L=1, nsteps=40)]                                  # N = 29
+    items[3].pulses[:, 1] = [2.0, 6.0, 10.0]
+    for p in items:
+        p.pulses[:, 2] = 4.0
+        p.pulses[:, 0] *= 1e5          # strong field: every ADO carries signal
+    return items
+
+
+@pytest.fixture(scope="module")
+def refs():
+    items = mixed_items()
+    return items, [O.twod(oproblem(p), nv=(1, 1, 1), want_rho=True) for p in items]
+
+
+def check(b, items, ref):
+    for i, p in enumerate(items):
+        Pr, rr = ref[i]
+        assert np.max(np.abs(rr[1:])) > 1e-12 * np.max(np.abs(rr[0])), "hierarchy not driven"
+        assert rel(b.output(i), Pr) < TOL, i
+        assert rel(b.rho(i), rr) < TOL, i
+
+
+def test_grouped_graph_and_per_launch_path_agree(ctx, refs):
+    items, ref = refs
+    b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)} for p in items])
+    b.run()
+    b.sync()
+    check(b, items, ref)
+    g_out = [b.output(i) for i in range(len(items))]
+    g_rho = [b.rho(i) for i in range(len(items))]
+    b.run_eager()
+    b.sync()
+    for i in range(len(items)):   # same kernels, same per-instance arithmetic: bitwise equal
+        assert np.array_equal(b.output(i), g_out[i])
+        assert np.array_equal(b.rho(i), g_rho[i])
+
+
+def test_every_launch_candidate_matches_oracle(ctx, refs):
+    items, ref = refs
+    b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)} for p in items])
+    cands = b.candidates()
+    kinds = {k for _, _, k, _ in cands}
+    assert {"stage_small", "stage_generic", "stage_mfma32"} <= kinds
+    assert {t for _, _, k, t in cands if k == "stage_small"} == {256, 512, 1024}
+    for g, c, kern, tile in cands:
+        print(f"group {g} candidate {c}: {kern} tile={tile}", flush=True)
+        b.select(g, c)
+        b.run()
+        b.sync()
+        check(b, items, ref)
+    with pytest.raises(E.EtoHeomError):
+        b.select(len({g for g, _, _, _ in cands}), 0)
+
+
+def test_autotune_picks_one_candidate_per_group_and_stays_correct(ctx, refs):
+    items, ref = refs
+    b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)} for p in items])
+    rep = b.autotune(2)
+    print(rep, flush=True)
+    lines = rep.splitlines()
+    groups = {int(l.split()[1]) for l in lines}
+    assert len(groups) == 4
+    for g in groups:
+        sel = [l for l in lines if int(l.split()[1]) == g and l.endswith("*")]
+        assert len(sel) == 1
+        ms = [float(l.split("ms_per_step=")[1].split()[0]) for l in lines if int(l.split()[1]) == g]
+        assert all(m > 0 for m in ms)
+    b.run()
+    b.sync()
+    check(b, items, ref)
