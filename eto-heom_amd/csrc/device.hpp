@@ -86,6 +86,16 @@ __device__ __forceinline__ void rk_load(const StageArgs& A, size_t gi, double2 r
     a = st == 0 ? make_double2(0.0, 0.0) : A.acc[gi];
 }
 
+// y and acc of element gi as the stage needs them, without touching the stage
+// input (stage 0 takes y = rin value at the store: no wait on that load here)
+__device__ __forceinline__ void rk_load_ya(const StageArgs& A, size_t gi, double2& y, double2& a) {
+    const int st = A.stage;
+    y = make_double2(0.0, 0.0);
+    a = make_double2(0.0, 0.0);
+    if (st == 1 || st == 2) y = A.y[gi];
+    if (st != 0) a = A.acc[gi];
+}
+
 // Coupling gathers of one ADO into LDS.  For bath j (site s = j + so), the
 // reference's one-hot products S_j rho_m and rho_m S_j (dynamics.cpp:143-179)
 // touch only row s and column s, so
@@ -119,6 +129,122 @@ __device__ __forceinline__ void gather_couplings(const StageArgs& A, const InstD
             const double2 wd = make_double2(-sg * al.y * y, sg * al.x * y);
             acc = cmad(acc, wu, xu);
             acc = cmad(acc, wd, xd);
+        }
+        cp[e] = acc;
+    }
+}
+
+// Two-phase form of gather_couplings for small-N tiles.  Phase 1 (gather_idx)
+// loads, for the first EC entries per thread and the first KMX modes, the
+// neighbour indices and weights; phase 2 (gather_data) the neighbour rows and
+// columns.  Every load is unconditional (clamped address, result selected), so the
+// compiler's wait counts are exact: vector loads retire in order and a load behind
+// a branch would make every later wait assume the shortest path.  Entries beyond
+// EC per thread and modes beyond KMX are added by gather_rest.
+template <int EC, int KMX>
+struct NbIdx {
+    int up[EC][KMX], dn[EC][KMX];
+    double x[EC][KMX], y[EC][KMX];
+    double2 al[EC][KMX];
+};
+
+template <int EC, int KMX>
+__device__ __forceinline__ void gather_idx(const StageArgs& A, const InstDev& I, int n0, int na, int tid, int nthr,
+                                           NbIdx<EC, KMX>& X) {
+    // raw values only: masking them here would make the thread wait for each load
+    const int N = I.N, K = I.K, Km = I.Km, d = I.d;
+    const int per = 2 * K * N, total = per * na;
+#pragma unroll
+    for (int r = 0; r < EC; ++r) {
+        const int e = tid + r * nthr;
+        const int ec = e < total ? e : total - 1;
+        const int g = ec / per, eg = ec - g * per;
+        const int j = eg / (2 * N), rc = (eg - j * 2 * N) / N;
+        const long long q0 = I.lk_off + (long long)(n0 + g) * d + j;
+#pragma unroll
+        for (int k = 0; k < KMX; ++k) {
+            const int kc = k < Km ? k : Km - 1;
+            const long long q = q0 + (long long)kc * K;
+            X.up[r][k] = A.nb_up[q];
+            X.dn[r][k] = A.nb_dn[q];
+            X.x[r][k] = A.cx[q];
+            X.y[r][k] = A.cy[q];
+            X.al[r][k] = rc == 0 ? A.alpha[I.al_off + j * Km + kc] : A.alpha_t[I.al_off + j * Km + kc];
+        }
+    }
+}
+
+template <int EC, int KMX>
+__device__ __forceinline__ void gather_data(const StageArgs& A, const InstDev& I, int na, int tid, int nthr,
+                                            const NbIdx<EC, KMX>& X, double2* cp) {
+    const int N = I.N, NN = N * N, K = I.K, Km = I.Km;
+    const int per = 2 * K * N, total = per * na;
+    double2 xu[EC][KMX], xd[EC][KMX];
+    int rcs[EC];
+#pragma unroll
+    for (int r = 0; r < EC; ++r) {
+        const int e = tid + r * nthr;
+        const int ec = e < total ? e : total - 1;
+        const int eg = ec - (ec / per) * per;
+        const int j = eg / (2 * N), rem = eg - j * 2 * N;
+        const int rc = rem / N, pos = rem - rc * N;
+        const int s = j + I.so;
+        const int ab = rc == 0 ? s * N + pos : pos * N + s;
+        rcs[r] = rc;
+#pragma unroll
+        for (int k = 0; k < KMX; ++k) {
+            const int mu = X.up[r][k], md = X.dn[r][k];
+            xu[r][k] = A.rin[I.state_off + (long long)(mu > 0 ? mu : 0) * NN + ab];
+            xd[r][k] = A.rin[I.state_off + (long long)(md > 0 ? md : 0) * NN + ab];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < EC; ++r) {
+        const int e = tid + r * nthr;
+        const double sg = rcs[r] == 0 ? A.gsign : -A.gsign;
+        double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int k = 0; k < KMX; ++k) {
+            // up: row i x, column -i x;  down: row i alpha y, column -i alpha~ y
+            const bool ok = e < total && k < Km;
+            const double x = ok && X.up[r][k] >= 0 ? X.x[r][k] : 0.0;
+            const double y = ok && X.dn[r][k] >= 0 ? X.y[r][k] : 0.0;
+            const double2 al = X.al[r][k];
+            acc = cmad(acc, make_double2(0.0, sg * x), xu[r][k]);
+            acc = cmad(acc, make_double2(-sg * al.y * y, sg * al.x * y), xd[r][k]);
+        }
+        if (e < total) cp[e] = acc;
+    }
+}
+
+// what gather_idx / gather_data leave out: modes k >= KMX of the first EC entries
+// per thread and every mode of the entries beyond; added into cp (same thread)
+template <int EC, int KMX>
+__device__ __forceinline__ void gather_rest(const StageArgs& A, const InstDev& I, int n0, int na, int tid, int nthr,
+                                            double2* cp) {
+    const int N = I.N, NN = N * N, K = I.K, Km = I.Km, d = I.d;
+    const int per = 2 * K * N, total = per * na;
+    for (int e = tid; e < total; e += nthr) {
+        const int r = (e - tid) / nthr;
+        const int k0 = r < EC ? KMX : 0;
+        if (k0 >= Km) continue;
+        const int g = e / per, eg = e - g * per;
+        const int j = eg / (2 * N), rem = eg - j * 2 * N;
+        const int rc = rem / N, pos = rem - rc * N;
+        const int s = j + I.so;
+        const int ab = rc == 0 ? s * N + pos : pos * N + s;
+        const long long row0 = I.lk_off + (long long)(n0 + g) * d;
+        const double sg = rc == 0 ? A.gsign : -A.gsign;
+        double2 acc = r < EC ? cp[e] : make_double2(0.0, 0.0);
+        for (int k = k0; k < Km; ++k) {
+            const long long q = row0 + j + k * K;
+            const int mu = A.nb_up[q], md = A.nb_dn[q];
+            const double2 xu = mu >= 0 ? A.rin[I.state_off + (long long)mu * NN + ab] : make_double2(0.0, 0.0);
+            const double2 xd = md >= 0 ? A.rin[I.state_off + (long long)md * NN + ab] : make_double2(0.0, 0.0);
+            const double x = A.cx[q], y = A.cy[q];
+            const double2 al = rc == 0 ? A.alpha[I.al_off + j * Km + k] : A.alpha_t[I.al_off + j * Km + k];
+            acc = cmad(acc, make_double2(0.0, sg * x), xu);
+            acc = cmad(acc, make_double2(-sg * al.y * y, sg * al.x * y), xd);
         }
         cp[e] = acc;
     }

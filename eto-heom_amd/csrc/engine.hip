@@ -48,39 +48,65 @@ __global__ __launch_bounds__(256) void stage_small(StageArgs A) {
     constexpr int NN = NT * NT;
     constexpr int G = NN >= GE ? 1 : GE / NN;
     constexpr int EPT = (G * NN + 255) / 256;
-    constexpr int CPMAX = G * 2 * NT * NT;   // K <= N
     __shared__ double2 sR[G * NN];
     __shared__ double2 sV[NN];
-    __shared__ double2 sC[CPMAX > 0 ? CPMAX : 1];
+    extern __shared__ double2 sC[];   // G * 2 K N coupling rows/columns (launch-time size)
     const int tile = blockIdx.x;
-    const InstDev& I = A.inst[A.tile_inst[tile]];
+    // a register copy: read once before any store (a reference would be re-read after them)
+    const InstDev I = A.inst[A.tile_inst[tile]];
     const int step = *A.step_ctr + A.step_off;
     if (step >= I.nsteps) return;
     const int a0 = A.tile_ado0[tile], na = A.tile_nado[tile];
     const long long base = I.state_off + (long long)a0 * NN;
     const int nel = na * NN;
-    // issue every load of the tile first: own elements, y/acc, V, coupling gathers
-    double2 rv[EPT], yv[EPT], av[EPT];
+    // Every load below is unconditional (clamped address, selected result): the
+    // neighbour indices and weights first, then own elements, y / acc, V, then the
+    // neighbour gathers, which therefore wait for the indices alone.
+    const int total = 2 * I.K * NT * na;
+    const bool has_cpl = total > 0 && I.Km > 0;
+    NbIdx<2, 2> nbx;
+    if (has_cpl) gather_idx<2, 2>(A, I, a0, na, threadIdx.x, 256, nbx);
+    const bool vreal = A.vreal[I.step_off + step] != 0;
+    // y and acc where the stage reads them, else the (just loaded, cached) input
+    const int st = A.stage;
+    const double2* yp = (st == 1 || st == 2) ? A.y : A.rin;
+    const double2* ap = st != 0 ? A.acc : A.rin;
+    double2 rv[EPT], yv[EPT], av[EPT], gv[EPT];
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
         const int e = threadIdx.x + q * 256;
-        rv[q] = e < nel ? A.rin[base + e] : make_double2(0.0, 0.0);
+        const int ec = e < nel ? e : nel - 1;
+        const size_t gi = (size_t)(base + ec);
+        rv[q] = A.rin[gi];
+        yv[q] = yp[gi];
+        av[q] = ap[gi];
+        gv[q] = A.damp[I.ado_off + a0 + ec / NN];
     }
+    // V into registers (written to LDS with the tile: a store right after its load
+    // would wait for every load issued before it)
+    constexpr int VPT = (NN + 255) / 256;
+    double2 vv[VPT];
 #pragma unroll
-    for (int q = 0; q < EPT; ++q) {
+    for (int q = 0; q < VPT; ++q) {
         const int e = threadIdx.x + q * 256;
-        if (e < nel) rk_load(A, (size_t)(base + e), rv[q], yv[q], av[q]);
+        vv[q] = A.V[I.v_off + (e < NN ? e : 0)];
     }
-    for (int e = threadIdx.x; e < NN; e += 256) sV[e] = A.V[I.v_off + e];
     const int cpn = 2 * I.K * NT;
-    gather_couplings(A, I, a0, na, sC, threadIdx.x, 256);
+    if (has_cpl) {
+        gather_data<2, 2>(A, I, na, threadIdx.x, 256, nbx, sC);
+        if (total > 2 * 256 || I.Km > 2) gather_rest<2, 2>(A, I, a0, na, threadIdx.x, 256, sC);
+    }
+#pragma unroll
+    for (int q = 0; q < VPT; ++q) {
+        const int e = threadIdx.x + q * 256;
+        if (e < NN) sV[e] = vv[q];
+    }
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
         const int e = threadIdx.x + q * 256;
         if (e < nel) sR[e] = rv[q];
     }
     __syncthreads();
-    const bool vreal = A.vreal[I.step_off + step] != 0;
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
         const int e = threadIdx.x + q * 256;
@@ -108,13 +134,13 @@ __global__ __launch_bounds__(256) void stage_small(StageArgs A) {
             }
         }
         double dr = ci, di = -cr;   // -i * comm
-        const double2 g = A.damp[I.ado_off + a0 + nl], r = rv[q];
+        const double2 g = gv[q], r = rv[q];
         dr = fma(g.x, r.x, fma(-g.y, r.y, dr));
         di = fma(g.x, r.y, fma(g.y, r.x, di));
         const double2 cpl = coupling_of(I, sC + nl * cpn, ra, cb);
         dr += cpl.x;
         di += cpl.y;
-        rk_store(A, I, (size_t)(base + e), yv[q], av[q], dr, di);
+        rk_store(A, I, (size_t)(base + e), st == 0 ? r : yv[q], st == 0 ? make_double2(0.0, 0.0) : av[q], dr, di);
     }
 }
 
@@ -122,7 +148,8 @@ __global__ __launch_bounds__(256) void stage_small(StageArgs A) {
 __global__ __launch_bounds__(256) void stage_generic(StageArgs A) {
     extern __shared__ double2 smem[];
     const int tile = blockIdx.x;
-    const InstDev& I = A.inst[A.tile_inst[tile]];
+    // a register copy: read once before any store (a reference would be re-read after them)
+    const InstDev I = A.inst[A.tile_inst[tile]];
     const int step = *A.step_ctr + A.step_off;
     if (step >= I.nsteps) return;
     const int N = I.N, NN = N * N;
@@ -190,7 +217,8 @@ __global__ __launch_bounds__(256, NP == 32 ? 4 : 1) void stage_mfma(StageArgs A)
     double* sVi = sVr + NP * LD;
     double2* sC = smem + (VLDS ? 2 : 1) * NP * LD;
     const int tile = blockIdx.x;
-    const InstDev& I = A.inst[A.tile_inst[tile]];
+    // a register copy: read once before any store (a reference would be re-read after them)
+    const InstDev I = A.inst[A.tile_inst[tile]];
     const int step = *A.step_ctr + A.step_off;
     if (step >= I.nsteps) return;
     const int N = I.N, NN = N * N;
@@ -647,11 +675,11 @@ int launch_cfg(const Group& g, const LaunchCfg& c, const StageArgs& base, hipStr
 #define EH_CASE(n)                                                                                 \
     case n:                                                                                        \
         if (c.ge == 256)                                                                           \
-            hipLaunchKernelGGL((stage_small<n, 256>), grid, block, 0, s, a);                       \
+            hipLaunchKernelGGL((stage_small<n, 256>), grid, block, c.shmem, s, a);                 \
         else if (c.ge == 1024)                                                                     \
-            hipLaunchKernelGGL((stage_small<n, 1024>), grid, block, 0, s, a);                      \
+            hipLaunchKernelGGL((stage_small<n, 1024>), grid, block, c.shmem, s, a);                \
         else                                                                                       \
-            hipLaunchKernelGGL((stage_small<n, 512>), grid, block, 0, s, a);                       \
+            hipLaunchKernelGGL((stage_small<n, 512>), grid, block, c.shmem, s, a);                 \
         break;
         EH_CASE(1) EH_CASE(2) EH_CASE(3) EH_CASE(4) EH_CASE(5) EH_CASE(6) EH_CASE(7) EH_CASE(8)
         EH_CASE(9) EH_CASE(10) EH_CASE(11) EH_CASE(12) EH_CASE(13) EH_CASE(14) EH_CASE(15) EH_CASE(16)
@@ -956,8 +984,10 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
                 c.shmem = (size_t)2 * 32 * 33 * sizeof(double2) + cpb;
             else if (nt == -64)
                 c.shmem = (size_t)64 * 65 * sizeof(double2) + cpb;
+            else if (nt > 0)
+                c.shmem = (size_t)G * cpb;   // the coupling rows/columns of G ADOs (the rest is static)
             else
-                c.shmem = nt > 0 ? 0 : (size_t)maxnn * sizeof(double2) + cpb;
+                c.shmem = (size_t)maxnn * sizeof(double2) + cpb;
             if ((int)c.shmem > ctx->lds_per_block) return;   // not a candidate on this device
             if (c.shmem > 65536) {
                 const void* fn = nt == -64 ? (const void*)stage_mfma<64, false>

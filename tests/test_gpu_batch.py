@@ -1,6 +1,7 @@
 """GPU batch engine: the per-group graph, the plain per-launch path, the launch
 candidates and the autotuner, each against the CPU oracle (<= 1e-10 normwise
-relative, FP64) on one batch that holds four N classes (N = 4, 7, 11, 29)."""
+relative, FP64) on one batch that holds four N classes (N = 4, 7, 11, 29) and an
+instance with three modes per bath."""
 import numpy as np
 import pytest
 
@@ -35,6 +36,12 @@ def mixed_items():
              configs.aggregate(4, 2, nsteps=90, taus=(20.0, 10.0, 30.0)),       # N = 11
              configs.heptamer(L=1, nsteps=40)]                                  # N = 29
     items[3].pulses[:, 1] = [2.0, 6.0, 10.0]
+    # three modes per bath (Km = 3): the modes beyond the kernels' two-phase gathers
+    km3 = configs.aggregate(2, 2, nsteps=110, taus=(10.0, 20.0, 30.0))
+    km3.alpha = np.concatenate([km3.alpha, 0.5 * km3.alpha[:, :1]], axis=1)
+    km3.alpha_t = np.concatenate([km3.alpha_t, 0.5 * km3.alpha_t[:, :1]], axis=1)
+    km3.gamma = np.concatenate([km3.gamma, 2.0 * km3.gamma[:, :1]], axis=1)
+    items.append(km3)
     for p in items:
         p.pulses[:, 2] = 4.0
         p.pulses[:, 0] *= 1e5          # strong field: every ADO carries signal
