@@ -24,6 +24,7 @@ struct InstDev {
     int so;                // site offset: bath j couples state j + so
     int al_off;            // offset of alpha / alpha~ (K*Km complex)
     double h2, h, hb[4];
+    long long col_off;     // complex offset of ADO 0 in the coupled-column copies (K*N per ADO)
 };
 
 struct StageArgs {
@@ -48,6 +49,13 @@ struct StageArgs {
     const double2* y;        // Y
     double2* acc;            // ACC
     double2* out;            // stage output
+    // Coupled-column copies: per ADO the K columns s = j + so of its matrix, each
+    // a contiguous N-vector ([ADO][j][row]).  A stage reads the neighbours' columns
+    // from cin (the copy of rin) and writes the copy of its output to cout, so the
+    // column gathers are contiguous instead of one cache line per element.  NULL:
+    // columns are gathered from rin directly.
+    const double2* cin;
+    double2* cout;
 };
 
 __device__ __forceinline__ double2 cmad(double2 acc, double2 a, double2 b) {
@@ -63,20 +71,34 @@ __device__ __forceinline__ double stage_hb(const InstDev& I, int st) {
 // RK4 stage update of one element from its RHS value k, with y / acc already loaded:
 //   stage 0: ACC = y + h/6 k, out = y + h/2 k;  1: ACC += h/3 k, out = y + h/2 k;
 //   2: ACC += h/3 k, out = y + h k;  3: out (= Y) = ACC + h/6 k
+// Element (row, col) of ADO m (instance-local index) also goes to the column copy.
 __device__ __forceinline__ void rk_store(const StageArgs& A, const InstDev& I, size_t gi, double2 y, double2 a,
-                                         double kr, double ki) {
+                                         double kr, double ki, int m, int row, int col) {
     const int st = A.stage;
     const double hb = stage_hb(I, st);
+    double2 o;
     if (st == 0) {
         A.acc[gi] = make_double2(fma(hb, kr, y.x), fma(hb, ki, y.y));
-        A.out[gi] = make_double2(fma(I.h2, kr, y.x), fma(I.h2, ki, y.y));
+        o = make_double2(fma(I.h2, kr, y.x), fma(I.h2, ki, y.y));
     } else if (st < 3) {
         const double hs = st == 1 ? I.h2 : I.h;
         A.acc[gi] = make_double2(fma(hb, kr, a.x), fma(hb, ki, a.y));
-        A.out[gi] = make_double2(fma(hs, kr, y.x), fma(hs, ki, y.y));
+        o = make_double2(fma(hs, kr, y.x), fma(hs, ki, y.y));
     } else {
-        A.out[gi] = make_double2(fma(hb, kr, a.x), fma(hb, ki, a.y));
+        o = make_double2(fma(hb, kr, a.x), fma(hb, ki, a.y));
     }
+    A.out[gi] = o;
+    const int jc = col - I.so;
+    if (A.cout && jc >= 0 && jc < I.K) A.cout[I.col_off + ((long long)m * I.K + jc) * I.N + row] = o;
+}
+
+// address of element (a, b) = (pos, s) [rc = 1] or (s, pos) [rc = 0] of neighbour mu
+// for bath j: columns from the column copy when the stage has one
+__device__ __forceinline__ const double2* nb_elem(const StageArgs& A, const InstDev& I, int mu, int j, int rc,
+                                                  int pos) {
+    const int N = I.N, s = j + I.so;
+    if (rc == 1 && A.cin) return A.cin + I.col_off + ((long long)mu * I.K + j) * N + pos;
+    return A.rin + I.state_off + (long long)mu * N * N + (rc == 0 ? s * N + pos : pos * N + s);
 }
 
 // y and acc of element gi as the stage needs them (stage 0 takes y = rin value)
@@ -105,22 +127,20 @@ __device__ __forceinline__ void rk_load_ya(const StageArgs& A, size_t gi, double
 // 2 Km gathers before it waits.
 __device__ __forceinline__ void gather_couplings(const StageArgs& A, const InstDev& I, int n0, int na, double2* cp,
                                                  int tid, int nthr) {
-    const int N = I.N, NN = N * N, K = I.K, Km = I.Km, d = I.d;
+    const int N = I.N, K = I.K, Km = I.Km, d = I.d;
     const int per = 2 * K * N;
     const int total = per * na;
     for (int e = tid; e < total; e += nthr) {
         const int g = e / per, eg = e - g * per;
         const int j = eg / (2 * N), rem = eg - j * 2 * N;
         const int rc = rem / N, pos = rem - rc * N;
-        const int s = j + I.so;
-        const int ab = rc == 0 ? s * N + pos : pos * N + s;
         const long long row0 = I.lk_off + (long long)(n0 + g) * d;
         double2 acc = make_double2(0.0, 0.0);
         for (int k = 0; k < Km; ++k) {
             const long long q = row0 + j + k * K;
             const int mu = A.nb_up[q], md = A.nb_dn[q];
-            const double2 xu = mu >= 0 ? A.rin[I.state_off + (long long)mu * NN + ab] : make_double2(0.0, 0.0);
-            const double2 xd = md >= 0 ? A.rin[I.state_off + (long long)md * NN + ab] : make_double2(0.0, 0.0);
+            const double2 xu = mu >= 0 ? *nb_elem(A, I, mu, j, rc, pos) : make_double2(0.0, 0.0);
+            const double2 xd = md >= 0 ? *nb_elem(A, I, md, j, rc, pos) : make_double2(0.0, 0.0);
             const double x = A.cx[q], y = A.cy[q];
             const double2 al = rc == 0 ? A.alpha[I.al_off + j * Km + k] : A.alpha_t[I.al_off + j * Km + k];
             // up:   row  i x,  column -i x;   down: row  i alpha y,  column -i alpha~ y
@@ -177,7 +197,7 @@ __device__ __forceinline__ void gather_idx(const StageArgs& A, const InstDev& I,
 template <int EC, int KMX>
 __device__ __forceinline__ void gather_data(const StageArgs& A, const InstDev& I, int na, int tid, int nthr,
                                             const NbIdx<EC, KMX>& X, double2* cp) {
-    const int N = I.N, NN = N * N, K = I.K, Km = I.Km;
+    const int N = I.N, K = I.K, Km = I.Km;
     const int per = 2 * K * N, total = per * na;
     double2 xu[EC][KMX], xd[EC][KMX];
     int rcs[EC];
@@ -188,14 +208,12 @@ __device__ __forceinline__ void gather_data(const StageArgs& A, const InstDev& I
         const int eg = ec - (ec / per) * per;
         const int j = eg / (2 * N), rem = eg - j * 2 * N;
         const int rc = rem / N, pos = rem - rc * N;
-        const int s = j + I.so;
-        const int ab = rc == 0 ? s * N + pos : pos * N + s;
         rcs[r] = rc;
 #pragma unroll
         for (int k = 0; k < KMX; ++k) {
             const int mu = X.up[r][k], md = X.dn[r][k];
-            xu[r][k] = A.rin[I.state_off + (long long)(mu > 0 ? mu : 0) * NN + ab];
-            xd[r][k] = A.rin[I.state_off + (long long)(md > 0 ? md : 0) * NN + ab];
+            xu[r][k] = *nb_elem(A, I, mu > 0 ? mu : 0, j, rc, pos);
+            xd[r][k] = *nb_elem(A, I, md > 0 ? md : 0, j, rc, pos);
         }
     }
 #pragma unroll
@@ -222,7 +240,7 @@ __device__ __forceinline__ void gather_data(const StageArgs& A, const InstDev& I
 template <int EC, int KMX>
 __device__ __forceinline__ void gather_rest(const StageArgs& A, const InstDev& I, int n0, int na, int tid, int nthr,
                                             double2* cp) {
-    const int N = I.N, NN = N * N, K = I.K, Km = I.Km, d = I.d;
+    const int N = I.N, K = I.K, Km = I.Km, d = I.d;
     const int per = 2 * K * N, total = per * na;
     for (int e = tid; e < total; e += nthr) {
         const int r = (e - tid) / nthr;
@@ -231,16 +249,14 @@ __device__ __forceinline__ void gather_rest(const StageArgs& A, const InstDev& I
         const int g = e / per, eg = e - g * per;
         const int j = eg / (2 * N), rem = eg - j * 2 * N;
         const int rc = rem / N, pos = rem - rc * N;
-        const int s = j + I.so;
-        const int ab = rc == 0 ? s * N + pos : pos * N + s;
         const long long row0 = I.lk_off + (long long)(n0 + g) * d;
         const double sg = rc == 0 ? A.gsign : -A.gsign;
         double2 acc = r < EC ? cp[e] : make_double2(0.0, 0.0);
         for (int k = k0; k < Km; ++k) {
             const long long q = row0 + j + k * K;
             const int mu = A.nb_up[q], md = A.nb_dn[q];
-            const double2 xu = mu >= 0 ? A.rin[I.state_off + (long long)mu * NN + ab] : make_double2(0.0, 0.0);
-            const double2 xd = md >= 0 ? A.rin[I.state_off + (long long)md * NN + ab] : make_double2(0.0, 0.0);
+            const double2 xu = mu >= 0 ? *nb_elem(A, I, mu, j, rc, pos) : make_double2(0.0, 0.0);
+            const double2 xd = md >= 0 ? *nb_elem(A, I, md, j, rc, pos) : make_double2(0.0, 0.0);
             const double x = A.cx[q], y = A.cy[q];
             const double2 al = rc == 0 ? A.alpha[I.al_off + j * Km + k] : A.alpha_t[I.al_off + j * Km + k];
             acc = cmad(acc, make_double2(0.0, sg * x), xu);

@@ -33,6 +33,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "engine_impl.hpp"
@@ -140,7 +141,8 @@ __global__ __launch_bounds__(256) void stage_small(StageArgs A) {
         const double2 cpl = coupling_of(I, sC + nl * cpn, ra, cb);
         dr += cpl.x;
         di += cpl.y;
-        rk_store(A, I, (size_t)(base + e), st == 0 ? r : yv[q], st == 0 ? make_double2(0.0, 0.0) : av[q], dr, di);
+        rk_store(A, I, (size_t)(base + e), st == 0 ? r : yv[q], st == 0 ? make_double2(0.0, 0.0) : av[q], dr, di,
+                 a0 + nl, ra, cb);
     }
 }
 
@@ -190,7 +192,7 @@ __global__ __launch_bounds__(256) void stage_generic(StageArgs A) {
         const double2 cpl = coupling_of(I, sC, ra, cb);
         double2 yv, av;
         rk_load(A, (size_t)(base + ab), r, yv, av);
-        rk_store(A, I, (size_t)(base + ab), yv, av, dr + cpl.x, di + cpl.y);
+        rk_store(A, I, (size_t)(base + ab), yv, av, dr + cpl.x, di + cpl.y, a0, ra, cb);
     }
 }
 
@@ -204,7 +206,7 @@ __global__ __launch_bounds__(256) void stage_generic(StageArgs A) {
 // All global loads of the workgroup (rho_n, V, coupling gathers, and y / acc of
 // the lane's output elements) are issued before the first wait.
 template <int NP, bool VLDS>
-__global__ __launch_bounds__(256, NP == 32 ? 4 : 1) void stage_mfma(StageArgs A) {
+__global__ __launch_bounds__(256, NP == 32 ? 4 : 2) void stage_mfma(StageArgs A) {
     constexpr int LD = NP + 1;
     constexpr int NB = NP / 16;
     constexpr int TPW = NB * NB / 4;
@@ -217,8 +219,11 @@ __global__ __launch_bounds__(256, NP == 32 ? 4 : 1) void stage_mfma(StageArgs A)
     double* sVi = sVr + NP * LD;
     double2* sC = smem + (VLDS ? 2 : 1) * NP * LD;
     const int tile = blockIdx.x;
-    // a register copy: read once before any store (a reference would be re-read after them)
-    const InstDev I = A.inst[A.tile_inst[tile]];
+    // N <= 32: a register copy of the instance descriptor, read once before any store
+    // (a reference would be re-read after them); N <= 64 keeps the reference: the
+    // copy costs it half its occupancy
+    using InstT = typename std::conditional<NP == 32, const InstDev, const InstDev&>::type;
+    InstT I = A.inst[A.tile_inst[tile]];
     const int step = *A.step_ctr + A.step_off;
     if (step >= I.nsteps) return;
     const int N = I.N, NN = N * N;
@@ -318,7 +323,7 @@ __global__ __launch_bounds__(256, NP == 32 ? 4 : 1) void stage_mfma(StageArgs A)
                 di = fma(g.x, x.y, fma(g.y, x.x, di));
                 const double2 cpl = coupling_of(I, sC, row, col);
                 const double2 y0 = A.stage == 0 ? x : yv[t][r];
-                rk_store(A, I, (size_t)(base + row * N + col), y0, av[t][r], dr + cpl.x, di + cpl.y);
+                rk_store(A, I, (size_t)(base + row * N + col), y0, av[t][r], dr + cpl.x, di + cpl.y, a0, row, col);
             }
         }
     }
@@ -631,15 +636,20 @@ __global__ __launch_bounds__(256, 2) void stage_mfma_pipe(StageArgs A, int ntile
     }
 }
 
-__global__ void reset_state(double2* y, long long total, const long long* ado0_off, int n_inst, int* ctr) {
+__global__ void reset_state(double2* y, long long total, double2* yc, long long ctotal, int* ctr) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long e = i; e < total; e += stride) y[e] = make_double2(0.0, 0.0);
+    for (long long e = i; e < ctotal; e += stride) yc[e] = make_double2(0.0, 0.0);
     if (i == 0) *ctr = 0;
 }
-__global__ void set_ground(double2* y, const long long* ado0_off, int n_inst) {
+// rho_0 = |0><0|; element (0, 0) lies in a coupled column when site 0 couples (so = 0)
+__global__ void set_ground(double2* y, double2* yc, const InstDev* inst, int n_inst) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n_inst) y[ado0_off[i]] = make_double2(1.0, 0.0);
+    if (i >= n_inst) return;
+    const InstDev& I = inst[i];
+    y[I.state_off] = make_double2(1.0, 0.0);
+    if (I.so == 0 && I.K > 0) yc[I.col_off] = make_double2(1.0, 0.0);
 }
 
 }  // namespace eh
@@ -667,6 +677,10 @@ static inline double2 d2(cplx c) { return make_double2(c.real(), c.imag()); }
 
 int launch_cfg(const Group& g, const LaunchCfg& c, const StageArgs& base, hipStream_t s) {
     StageArgs a = base;
+    if (!c.cols) {   // this configuration gathers columns from rin
+        a.cin = nullptr;
+        a.cout = nullptr;
+    }
     a.tile_inst += c.tile_begin;
     a.tile_ado0 += c.tile_begin;
     a.tile_nado += c.tile_begin;
@@ -687,7 +701,7 @@ int launch_cfg(const Group& g, const LaunchCfg& c, const StageArgs& base, hipStr
         case -32: hipLaunchKernelGGL((stage_mfma<32, true>), grid, block, c.shmem, s, a); break;
         case -64: hipLaunchKernelGGL((stage_mfma<64, false>), grid, block, c.shmem, s, a); break;
         case -33:
-            // the RK coefficients of this stage (one time step for the whole group)
+            // the RK coefficients of this stage are arguments (one time step per group)
             hipLaunchKernelGGL(stage_mfma_pipe<2>, dim3(std::min(c.tile_count, g.persist)), block, c.shmem, s, a,
                                c.tile_count, g.maxkn, g.h2, g.h, g.hb[a.stage]);
             break;
@@ -703,6 +717,8 @@ StageArgs stage_io(const StageArgs& sa, const BatchImpl* im, int k, int st) {
     a.stage = st;
     a.rin = st == 0 ? im->Y.p : (st == 2 ? im->S2.p : im->S1.p);
     a.out = st == 3 ? im->Y.p : (st == 1 ? im->S2.p : im->S1.p);
+    a.cin = st == 0 ? im->Yc.p : (st == 2 ? im->S2c.p : im->S1c.p);
+    a.cout = st == 3 ? im->Yc.p : (st == 1 ? im->S2c.p : im->S1c.p);
     return a;
 }
 
@@ -744,6 +760,17 @@ PrologueArgs prologue_args(const BatchImpl* im) {
     pa.P = im->P.p;
     pa.readout = 1;
     return pa;
+}
+
+// every state to rho_0 = |0><0| (and its column copy), step counter to 0
+static void launch_reset(Batch* b) {
+    BatchImpl* im = b->impl;
+    const long long total = std::max((long long)b->total_elems, (long long)im->Yc.n);
+    const int blocks = (int)std::min<long long>(std::max<long long>(1, (total + 255) / 256), 65536);
+    hipLaunchKernelGGL(reset_state, dim3(blocks), dim3(256), 0, im->stream, im->Y.p, (long long)b->total_elems,
+                       im->Yc.p, (long long)im->Yc.n, im->step_ctr.p);
+    hipLaunchKernelGGL(set_ground, dim3((b->n_inst + 255) / 256), dim3(256), 0, im->stream, im->Y.p, im->Yc.p,
+                       im->inst.p, b->n_inst);
 }
 
 // The forward graph: `chunk` steps, each group on its own branch (instances of
@@ -806,7 +833,7 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
     std::vector<uint8_t> hvreal;
     std::vector<double2> hH, hXp, hfield, hR, hal, halt, hdamp;
     std::vector<long long> hado0;
-    int64_t state_off = 0, p_off = 0, lk_off = 0;
+    int64_t state_off = 0, p_off = 0, lk_off = 0, col_off = 0;
     int ado_off = 0, v_off = 0, r_off = 0, step_off = 0, al_off = 0;
     // instances by N class: 1..16 (each N its own class), 17 (16 < N <= 32), 18 (32 < N <= 64)
     std::vector<std::vector<int>> by_class(19);
@@ -897,6 +924,8 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
         d.Km = sys.Km;
         d.so = sys.site_offset;
         d.al_off = al_off;
+        d.col_off = col_off;
+        col_off += (int64_t)M * sys.K * N;
         d.h2 = s.h2;
         d.h = s.h;
         for (int k = 0; k < 4; k++) d.hb[k] = s.hb[k];
@@ -970,10 +999,11 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
             return r;
         };
         const size_t cpb = (size_t)maxkn * sizeof(double2);   // coupling rows/columns
-        auto add = [&](int nt, int ge) {
+        auto add = [&](int nt, int ge, bool cols) {
             LaunchCfg c;
             c.nt = nt;
             c.ge = ge;
+            c.cols = cols;
             const int G = nt > 0 ? (maxnn >= ge ? 1 : ge / maxnn) : 1;
             const std::pair<int, int> r = tiles(G);
             c.tile_begin = r.first;
@@ -997,15 +1027,18 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
             }
             g.cand.push_back(c);
         };
+        // column copies: measured to pay off for N > 16 (long strided columns), not below
         if (cls <= 16) {
-            add(cls, 512);   // the base configuration: the adjoint's small kernels tile as it does
-            add(cls, 256);
-            add(cls, 1024);
-            add(0, 0);
+            add(cls, 512, false);   // the base configuration: the adjoint's small kernels tile as it does
+            add(cls, 256, false);
+            add(cls, 1024, false);
+            add(cls, 512, true);
+            add(0, 0, false);
         } else {
-            if (use_mfma) add(cls == 17 ? -32 : -64, 0);
-            add(0, 0);
-            if (pipe_ok) add(-33, 0);
+            if (use_mfma) add(cls == 17 ? -32 : -64, 0, true);
+            add(0, 0, true);
+            if (use_mfma) add(cls == 17 ? -32 : -64, 0, false);
+            if (pipe_ok) add(-33, 0, false);
         }
         if (g.cand.empty() || (cls <= 16 && g.cand[0].nt != cls))
             return fail(EH_ESHAPE, "state tile exceeds LDS");
@@ -1030,7 +1063,9 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
         return rc;
     if ((rc = im->step_ctr.alloc(1)) || (rc = im->V.alloc(hH.size())) || (rc = im->P.alloc((size_t)p_off)) ||
         (rc = im->Y.alloc((size_t)state_off)) || (rc = im->ACC.alloc((size_t)state_off)) ||
-        (rc = im->S1.alloc((size_t)state_off)) || (rc = im->S2.alloc((size_t)state_off)))
+        (rc = im->S1.alloc((size_t)state_off)) || (rc = im->S2.alloc((size_t)state_off)) ||
+        (rc = im->Yc.alloc((size_t)col_off)) || (rc = im->S1c.alloc((size_t)col_off)) ||
+        (rc = im->S2c.alloc((size_t)col_off)))
         return rc;
     im->hinst = hinst;
     im->hup = std::move(hup);
@@ -1082,12 +1117,7 @@ int batch_run(Batch* b) {
     if (!b || !b->impl) return fail(EH_EINVAL, "batch_run: null batch");
     BatchImpl* im = b->impl;
     HIPCHK(hipSetDevice(b->ctx->device));
-    const long long total = b->total_elems;
-    const int blocks = (int)std::min<long long>(std::max<long long>(1, (total + 255) / 256), 65536);
-    hipLaunchKernelGGL(reset_state, dim3(blocks), dim3(256), 0, im->stream, im->Y.p, total, im->ado0_off.p,
-                       b->n_inst, im->step_ctr.p);
-    hipLaunchKernelGGL(set_ground, dim3((b->n_inst + 255) / 256), dim3(256), 0, im->stream, im->Y.p,
-                       im->ado0_off.p, b->n_inst);
+    launch_reset(b);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(im->ev0, im->stream));
     if (im->adj) {
@@ -1129,12 +1159,7 @@ int batch_profile(Batch* b, int nsteps, double* out) {
     if (!b || !b->impl || nsteps <= 0 || !out) return fail(EH_EINVAL, "batch_profile: bad arguments");
     BatchImpl* im = b->impl;
     HIPCHK(hipSetDevice(b->ctx->device));
-    const long long total = b->total_elems;
-    const int blocks = (int)std::min<long long>(std::max<long long>(1, (total + 255) / 256), 65536);
-    hipLaunchKernelGGL(reset_state, dim3(blocks), dim3(256), 0, im->stream, im->Y.p, total, im->ado0_off.p,
-                       b->n_inst, im->step_ctr.p);
-    hipLaunchKernelGGL(set_ground, dim3((b->n_inst + 255) / 256), dim3(256), 0, im->stream, im->Y.p,
-                       im->ado0_off.p, b->n_inst);
+    launch_reset(b);
     std::vector<hipEvent_t> ev((size_t)6 * nsteps);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     const StageArgs sa = stage_args(im);
@@ -1179,12 +1204,7 @@ int batch_run_eager(Batch* b) {
     BatchImpl* im = b->impl;
     if (im->adj) return fail(EH_EINVAL, "batch_run_eager: the adjoint needs the checkpointed run (eh_batch_run)");
     HIPCHK(hipSetDevice(b->ctx->device));
-    const long long total = b->total_elems;
-    const int blocks = (int)std::min<long long>(std::max<long long>(1, (total + 255) / 256), 65536);
-    hipLaunchKernelGGL(reset_state, dim3(blocks), dim3(256), 0, im->stream, im->Y.p, total, im->ado0_off.p,
-                       b->n_inst, im->step_ctr.p);
-    hipLaunchKernelGGL(set_ground, dim3((b->n_inst + 255) / 256), dim3(256), 0, im->stream, im->Y.p,
-                       im->ado0_off.p, b->n_inst);
+    launch_reset(b);
     HIPCHK(hipEventRecord(im->ev0, im->stream));
     const StageArgs sa = stage_args(im);
     const PrologueArgs pa = prologue_args(im);
@@ -1214,12 +1234,7 @@ int batch_autotune(Batch* b, int steps) {
     BatchImpl* im = b->impl;
     if (im->adj) return fail(EH_EINVAL, "batch_autotune: tune before enabling the adjoint");
     HIPCHK(hipSetDevice(b->ctx->device));
-    const long long total = b->total_elems;
-    const int blocks = (int)std::min<long long>(std::max<long long>(1, (total + 255) / 256), 65536);
-    hipLaunchKernelGGL(reset_state, dim3(blocks), dim3(256), 0, im->stream, im->Y.p, total, im->ado0_off.p,
-                       b->n_inst, im->step_ctr.p);
-    hipLaunchKernelGGL(set_ground, dim3((b->n_inst + 255) / 256), dim3(256), 0, im->stream, im->Y.p,
-                       im->ado0_off.p, b->n_inst);
+    launch_reset(b);
     const StageArgs sa = stage_args(im);
     const PrologueArgs pa = prologue_args(im);
     hipLaunchKernelGGL(prologue, dim3(b->n_inst), dim3(256), 0, im->stream, pa);   // V of step 0
@@ -1281,8 +1296,8 @@ std::string batch_tuning_report(const Batch* b) {
                              : (x.nt == -33 ? "stage_mfma_pipe" : "stage_generic")));
             char line[256];
             std::snprintf(line, sizeof(line),
-                          "group %zu cand %zu maxN^2=%d instances=%d %s tile=%d tiles=%d ms_per_step=%.6f%s\n", gi, c,
-                          g.maxnn, g.inst_count, kn, x.nt > 0 ? x.ge : 0, x.tile_count,
+                          "group %zu cand %zu maxN^2=%d instances=%d %s tile=%d tiles=%d cols=%d ms_per_step=%.6f%s\n",
+                          gi, c, g.maxnn, g.inst_count, kn, x.nt > 0 ? x.ge : 0, x.tile_count, x.cols ? 1 : 0,
                           g.cand_ms.empty() ? -1.0 : g.cand_ms[c], (int)c == g.sel ? " *" : "");
             out += line;
         }

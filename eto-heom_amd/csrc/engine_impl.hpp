@@ -54,6 +54,7 @@ struct DevBuf {
 struct LaunchCfg {
     int nt;                   // 1..16: stage_small<nt>; -32 / -64: stage_mfma; -33: stage_mfma_pipe; 0: stage_generic
     int ge;                   // stage_small: state elements per tile (256, 512 or 1024)
+    bool cols;                // keep coupled-column copies (contiguous column gathers)
     int tile_begin, tile_count;
     size_t shmem;
 };
@@ -85,6 +86,7 @@ struct BatchImpl {
     DevBuf<uint8_t> vreal;
     DevBuf<double> cx, cy;
     DevBuf<double2> V, H, Xp, field, R, P, alpha, alpha_t, damp, Y, ACC, S1, S2;
+    DevBuf<double2> Yc, S1c, S2c;        // coupled-column copies of Y, S1, S2
     DevBuf<long long> ado0_off;
     DevBuf<int> ginst;                   // instance indices, grouped
     std::vector<Group> groups;
