@@ -30,6 +30,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -47,13 +48,16 @@ struct AdjArgs {
     const double* fcy;
     const uint8_t* ado;      // multi-indices in (ADO, p) layout
     const double2* field;    // per step e_q
-    double2* gacc;           // per instance accumulators
+    double2* gacc;           // per instance accumulators, ncopy copies each
     const long long* gacc_off;
+    int ncopy;               // tile t adds into copy t % ncopy (spreads the atomics)
+    long long traj;          // trajectory mode: Yf moves by step * traj elements
 };
 
 // layout of one instance's accumulator block (complex entries)
 __host__ __device__ inline long long gacc_size(int NN, int d) { return 5LL * NN + 4LL * d; }
 // [0, NN) Gt | [NN, 4NN) G_q weighted by e_q | [4NN, 5NN) GX3 | Sg[d] | U[d] | A[d] | B[d]
+// Each instance holds ncopy such blocks; the host sums them after the backward pass.
 
 __device__ __forceinline__ void atomic_add2(double2* p, double2 v) {
     atomicAdd(&p->x, v.x);
@@ -123,6 +127,66 @@ __device__ void reduce_bath(const AdjArgs& J, const InstDev& I, int n0, int na, 
     }
 }
 
+// The coupling-parameter reductions with one thread per (ADO, bath j, row/column,
+// mode k): it sums conj(kb) rho over the N positions of row / column s = j + so
+// of the tile's ADO (kb from LDS) against the forward stage input of the up and
+// down neighbours, then adds its two results into the LDS sums (x: U_p;  y: A_p
+// for rows, B_p for columns; p = j + k K).
+template <int NT>
+__device__ __forceinline__ void reduce_bath_rows(const AdjArgs& J, const InstDev& I, int n0, int na,
+                                                 const double2* sKb, double fs, double* sU, int tid, int nthr) {
+    constexpr int NN = NT * NT;
+    const int K = I.K, Km = I.Km, d = I.d;
+    const int nt = na * K * 2 * Km;
+    for (int t = tid; t < nt; t += nthr) {
+        const int k = t % Km, r1 = t / Km;
+        const int rc = r1 % 2, r2 = r1 / 2;
+        const int j = r2 % K, g = r2 / K;
+        const int s = j + I.so, p = j + k * K;
+        const long long q = I.lk_off + (long long)(n0 + g) * d + p;
+        const int mu = J.fnb_up[q], md = J.fnb_dn[q];
+        const double x = J.fcx[q], y = J.fcy[q];
+        const double2* kb = sKb + g * NN;
+        const double2* Yu = J.Yf + I.state_off + (long long)(mu > 0 ? mu : 0) * NN;
+        const double2* Yd = J.Yf + I.state_off + (long long)(md > 0 ? md : 0) * NN;
+        // in chunks of up to 8 positions: all loads of a chunk in flight together
+        constexpr int CH = NT < 8 ? NT : 8;
+        double2 tu = make_double2(0.0, 0.0), td = make_double2(0.0, 0.0);
+#pragma unroll 1
+        for (int p0 = 0; p0 < NT; p0 += CH) {
+            double2 yu[CH], yd[CH];
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                const int pos = p0 + c < NT ? p0 + c : NT - 1;
+                const int ab = rc == 0 ? s * NT + pos : pos * NT + s;
+                yu[c] = Yu[ab];
+                yd[c] = Yd[ab];
+            }
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                if (p0 + c >= NT) break;
+                const int pos = p0 + c;
+                const double2 kv = kb[rc == 0 ? s * NT + pos : pos * NT + s];
+                const double2 a = cconj_mul(kv, yu[c]), e = cconj_mul(kv, yd[c]);
+                tu.x += a.x;
+                tu.y += a.y;
+                td.x += e.x;
+                td.y += e.y;
+            }
+        }
+        const double sgn = (rc == 0 ? 1.0 : -1.0) * fs;
+        if (mu >= 0) {   // x (+-i) conj(kb) yu
+            atomicAdd(&sU[2 * p], -sgn * x * tu.y);
+            atomicAdd(&sU[2 * p + 1], sgn * x * tu.x);
+        }
+        if (md >= 0) {
+            double* dst = sU + 2 * d * (rc == 0 ? 1 : 2);   // A_p (row) or B_p (column)
+            atomicAdd(&dst[2 * p], -sgn * y * td.y);
+            atomicAdd(&dst[2 * p + 1], sgn * y * td.x);
+        }
+    }
+}
+
 template <int NT>
 __global__ __launch_bounds__(256) void adj_small(AdjArgs J) {
     const StageArgs& A = J.S;
@@ -140,37 +204,61 @@ __global__ __launch_bounds__(256) void adj_small(AdjArgs J) {
     __shared__ double sU[3 * 2 * DMAX];
     const int tile = blockIdx.x;
     const int ii = A.tile_inst[tile];
-    const InstDev& I = A.inst[ii];
+    // a register copy: read once before any store (a reference would be re-read after them)
+    const InstDev I = A.inst[ii];
     const int step = *A.step_ctr + A.step_off;
     if (step < 0 || step >= I.nsteps) return;
+    J.Yf += (long long)step * J.traj;
     const int a0 = A.tile_ado0[tile], na = A.tile_nado[tile];
     const long long base = I.state_off + (long long)a0 * NN;
     const int nel = na * NN;
     const int st = A.stage;
-    double2 rv[EPT], lv[EPT], av[EPT];
+    // loads first, all unconditional (clamped address, selected result): the
+    // neighbour tables, then the tile's inputs, then the neighbour gathers
+    const int total = 2 * I.K * NT * na;
+    const bool has_cpl = total > 0 && I.Km > 0;
+    NbIdx<2, 2> nbx;
+    if (has_cpl) gather_idx<2, 2>(A, I, a0, na, threadIdx.x, 256, nbx);
+    const bool vreal = A.vreal[I.step_off + step] != 0;
+    const double2* lp = (st == 1 || st == 2) ? A.y : A.rin;
+    const double2* ap = st > 0 ? A.acc : A.rin;
+    double2 rv[EPT], lv[EPT], av[EPT], yf[EPT], gv[EPT];
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
         const int e = threadIdx.x + q * 256;
-        rv[q] = make_double2(0.0, 0.0);
-        lv[q] = make_double2(0.0, 0.0);
-        av[q] = make_double2(0.0, 0.0);
+        const int ec = e < nel ? e : nel - 1;
+        rv[q] = A.rin[base + ec];
+        lv[q] = lp[base + ec];
+        av[q] = ap[base + ec];
+        yf[q] = J.Yf[base + ec];
+        gv[q] = A.damp[I.ado_off + a0 + ec / NN];
+    }
+    constexpr int VPT = (NN + 255) / 256;
+    double2 vv[VPT];
+#pragma unroll
+    for (int q = 0; q < VPT; ++q) {
+        const int e = threadIdx.x + q * 256;
+        vv[q] = A.V[I.v_off + (e < NN ? e : 0)];
+    }
+    if (has_cpl) {
+        gather_data<2, 2>(A, I, na, threadIdx.x, 256, nbx, sC);
+        if (total > 2 * 256 || I.Km > 2) gather_rest<2, 2>(A, I, a0, na, threadIdx.x, 256, sC);
+    }
+    for (int e = threadIdx.x; e < 6 * I.d; e += 256) sU[e] = 0.0;
+#pragma unroll
+    for (int q = 0; q < VPT; ++q) {
+        const int e = threadIdx.x + q * 256;
+        if (e < NN) sV[e] = vv[q];
+    }
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+        const int e = threadIdx.x + q * 256;
         if (e < nel) {
-            rv[q] = A.rin[base + e];
-            if (st == 1 || st == 2) lv[q] = A.y[base + e];
-            if (st > 0) av[q] = A.acc[base + e];
-            sY[e] = J.Yf[base + e];
+            sR[e] = rv[q];
+            sY[e] = yf[q];
         }
     }
-    for (int e = threadIdx.x; e < NN; e += 256) sV[e] = A.V[I.v_off + e];
-    for (int e = threadIdx.x; e < 6 * I.d; e += 256) sU[e] = 0.0;
-    gather_couplings(A, I, a0, na, sC, threadIdx.x, 256);
-#pragma unroll
-    for (int q = 0; q < EPT; ++q) {
-        const int e = threadIdx.x + q * 256;
-        if (e < nel) sR[e] = rv[q];
-    }
     __syncthreads();
-    const bool vreal = A.vreal[I.step_off + step] != 0;
     const double fs = st == 0 ? I.h / 6.0 : 1.0;   // kb = fs * stage input
     const int cpn = 2 * I.K * NT;
 #pragma unroll
@@ -213,18 +301,20 @@ __global__ __launch_bounds__(256) void adj_small(AdjArgs J) {
         const double2 qv = cconj_mul(R[ab], Y[ab]);
         sQ[e] = make_double2(fs * qv.x, fs * qv.y);
         double ur = -ci, ui = cr;   // +i * C'
-        const double2 g = A.damp[I.ado_off + a0 + nl], r = R[ab];
+        const double2 g = gv[q], r = R[ab];
         ur = fma(g.x, r.x, fma(-g.y, r.y, ur));
         ui = fma(g.x, r.y, fma(g.y, r.x, ui));
         const double2 cpl = coupling_of(I, sC + nl * cpn, ra, cb);
         const double2 lam = st == 0 ? r : lv[q];
-        rk_store_adj(A, I, (size_t)(base + e), lam, av[q], ur + cpl.x, ui + cpl.y);
+        rk_store_adj(A, I, (size_t)(base + e), lam, st == 0 ? make_double2(0.0, 0.0) : av[q], ur + cpl.x,
+                     ui + cpl.y);
     }
-    reduce_bath(J, I, a0, na, sR, NN, fs, sU, threadIdx.x, 256);
+    reduce_bath_rows<NT>(J, I, a0, na, sR, fs, sU, threadIdx.x, 256);
     __syncthreads();
-    double2* acc = J.gacc + J.gacc_off[ii];
+    double2* acc = J.gacc + J.gacc_off[ii] + (long long)(tile % J.ncopy) * gacc_size(NN, I.d);
     const double2* ef = J.field + 3 * (size_t)(I.step_off + step);
     const double2 e0 = ef[0], e1 = ef[1], e2 = ef[2];
+    const bool field_on = e0.x != 0.0 || e0.y != 0.0 || e1.x != 0.0 || e1.y != 0.0 || e2.x != 0.0 || e2.y != 0.0;
     for (int ab = threadIdx.x; ab < NN; ab += 256) {
         double2 gs = make_double2(0.0, 0.0);
         for (int n = 0; n < na; ++n) {
@@ -232,9 +322,11 @@ __global__ __launch_bounds__(256) void adj_small(AdjArgs J) {
             gs.y += sG[n * NN + ab].y;
         }
         atomic_add2(acc + ab, gs);
-        atomic_add2(acc + NN + ab, cmad(make_double2(0.0, 0.0), e0, gs));
-        atomic_add2(acc + 2 * NN + ab, cmad(make_double2(0.0, 0.0), e1, gs));
-        atomic_add2(acc + 3 * NN + ab, cmad(make_double2(0.0, 0.0), e2, gs));
+        if (field_on) {   // G_q = sum e_q Gt: nothing to add outside the pulse windows
+            atomic_add2(acc + NN + ab, cmad(make_double2(0.0, 0.0), e0, gs));
+            atomic_add2(acc + 2 * NN + ab, cmad(make_double2(0.0, 0.0), e1, gs));
+            atomic_add2(acc + 3 * NN + ab, cmad(make_double2(0.0, 0.0), e2, gs));
+        }
     }
     const int d = I.d;
     for (int t = threadIdx.x; t < na * d; t += 256) {
@@ -260,9 +352,10 @@ __global__ __launch_bounds__(256) void adj_generic(AdjArgs J) {
     extern __shared__ double2 smem[];
     const int tile = blockIdx.x;
     const int ii = A.tile_inst[tile];
-    const InstDev& I = A.inst[ii];
+    const InstDev I = A.inst[ii];   // register copy (read before any store)
     const int step = *A.step_ctr + A.step_off;
     if (step < 0 || step >= I.nsteps) return;
+    J.Yf += (long long)step * J.traj;
     const int N = I.N, NN = N * N, d = I.d;
     double2* sR = smem;
     double2* sY = smem + NN;
@@ -283,9 +376,10 @@ __global__ __launch_bounds__(256) void adj_generic(AdjArgs J) {
     const bool vreal = A.vreal[I.step_off + step] != 0;
     const double fs = st == 0 ? I.h / 6.0 : 1.0;
     const double2 g = A.damp[I.ado_off + a0];
-    double2* acc = J.gacc + J.gacc_off[ii];
+    double2* acc = J.gacc + J.gacc_off[ii] + (long long)(tile % J.ncopy) * gacc_size(NN, d);
     const double2* ef = J.field + 3 * (size_t)(I.step_off + step);
     const double2 e0 = ef[0], e1 = ef[1], e2 = ef[2];
+    const bool field_on = e0.x != 0.0 || e0.y != 0.0 || e1.x != 0.0 || e1.y != 0.0 || e2.x != 0.0 || e2.y != 0.0;
     double2 qsum = make_double2(0.0, 0.0);
     for (int ab = threadIdx.x; ab < NN; ab += blockDim.x) {
         const int ra = ab / N, cb = ab - ra * N;
@@ -310,9 +404,11 @@ __global__ __launch_bounds__(256) void adj_generic(AdjArgs J) {
         }
         gt = make_double2(fs * gt.x, fs * gt.y);
         atomic_add2(acc + ab, gt);
-        atomic_add2(acc + NN + ab, cmad(make_double2(0.0, 0.0), e0, gt));
-        atomic_add2(acc + 2 * NN + ab, cmad(make_double2(0.0, 0.0), e1, gt));
-        atomic_add2(acc + 3 * NN + ab, cmad(make_double2(0.0, 0.0), e2, gt));
+        if (field_on) {
+            atomic_add2(acc + NN + ab, cmad(make_double2(0.0, 0.0), e0, gt));
+            atomic_add2(acc + 2 * NN + ab, cmad(make_double2(0.0, 0.0), e1, gt));
+            atomic_add2(acc + 3 * NN + ab, cmad(make_double2(0.0, 0.0), e2, gt));
+        }
         const double2 qv = cconj_mul(sR[ab], sY[ab]);
         qsum.x += fs * qv.x;
         qsum.y += fs * qv.y;
@@ -378,6 +474,7 @@ struct AdjReadoutArgs {
     double2* lam;
     double2* gacc;
     const long long* gacc_off;
+    long long traj;        // trajectory mode: y moves by step * traj elements
 };
 
 // l[ADO 0][j][k] += conj(X3[k][j]) g_i ;  dL/dX3[k][j] += conj(rho_0[j][k]) g_i
@@ -399,7 +496,7 @@ __global__ __launch_bounds__(256) void adj_readout(AdjReadoutArgs P) {
         l.x += t.x;
         l.y += t.y;
         P.lam[I.state_off + q] = l;
-        const double2 r = P.y[I.state_off + q];
+        const double2 r = P.y[(long long)step * P.traj + I.state_off + q];
         const double2 u = cconj_mul(r, g);
         acc[k * N + j].x += u.x;
         acc[k * N + j].y += u.y;
@@ -422,6 +519,14 @@ __global__ void ckpt_load(const double2* ckpt, double2* dst, long long total, co
         dst[e] = src[e];
 }
 
+// trajectory mode: the state each instance ended in (slot 4 nsteps) -> Y
+__global__ void traj_final(const double2* traj, double2* y, const InstDev* inst, long long slot_elems) {
+    const InstDev& I = inst[blockIdx.x];
+    const long long n = (long long)I.M * I.N * I.N;
+    const double2* src = traj + 4LL * I.nsteps * slot_elems + I.state_off;
+    for (long long e = threadIdx.x; e < n; e += blockDim.x) y[I.state_off + e] = src[e];
+}
+
 __global__ void fill_zero(double2* p, long long n) {
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x)
         p[e] = make_double2(0.0, 0.0);
@@ -435,8 +540,11 @@ __global__ void set_ctr(int* ctr, int v) {
 
 struct AdjState {
     int C = 0;                 // checkpoint interval (steps per segment)
+    int ncopy = 32;            // accumulator copies per instance
     int nseg = 0;
     long long total = 0;       // state elements of the batch
+    bool trajm = false;        // trajectory mode: every stage input of the run is kept, no replay
+    DevBuf<double2> traj;      // trajectory mode: (4 max_steps + 1) states
     DevBuf<double2> ckpt;      // (nseg + 1) states
     DevBuf<double2> stg;       // (C + 1) x 4 stage inputs of the current segment
     DevBuf<double2> acc_f;     // forward accumulator scratch of the replay
@@ -492,6 +600,7 @@ int batch_enable_adjoint(Batch* b, int C) {
     HIPCHK(hipSetDevice(b->ctx->device));
     std::unique_ptr<AdjState> a(new AdjState());
     a->C = std::min(C, std::max(1, b->max_steps));
+    if (const char* ev = std::getenv("ETOHEOM_ADJ_COPIES")) a->ncopy = std::max(1, std::atoi(ev));
     a->nseg = (b->max_steps + a->C - 1) / a->C;
     a->total = b->total_elems;
     // adjoint tables: up <- down, down <- up, with the coefficients of the partner link
@@ -516,7 +625,7 @@ int batch_enable_adjoint(Batch* b, int C) {
                 if (adn[q] >= 0) acy[q] = std::sqrt((mp + 1) / aabs);    // y of the link (m + e_p) -> m
             }
         goff.push_back(gtot);
-        gtot += gacc_size(I.N * I.N, I.d);
+        gtot += (long long)a->ncopy * gacc_size(I.N * I.N, I.d);
         if (I.d > 64 && I.N <= 16) return fail(EH_ERANGE, "adjoint: hierarchy dimension above 64");
     }
     for (size_t e = 0; e < aal.size(); e++) {
@@ -527,26 +636,50 @@ int batch_enable_adjoint(Batch* b, int C) {
     a->h_gacc_off = goff;
     int rc;
     const size_t T = (size_t)a->total;
+    // Trajectory mode when every stage input of the run fits in half of the free
+    // HBM: the forward writes them into per-step slots (its stage outputs are the
+    // next stage's inputs, so this costs no extra traffic) and the backward needs
+    // no recomputation.  Otherwise checkpoints every C steps and a replay per segment.
+    {
+        size_t fr = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fr, &tot));
+        const double need = (double)T * sizeof(double2) * (4.0 * b->max_steps + 1.0);
+        const char* ev = std::getenv("ETOHEOM_ADJ_REPLAY");
+        a->trajm = !(ev && ev[0] == '1') && need < 0.5 * (double)fr;
+    }
     if ((rc = a->nbA_up.upload(aup)) || (rc = a->nbA_dn.upload(adn)) || (rc = a->cxA.upload(acx)) ||
         (rc = a->cyA.upload(acy)) || (rc = a->alA.upload(aal)) || (rc = a->altA.upload(aalt)) ||
         (rc = a->dampA.upload(adamp)) || (rc = a->ado.upload(im->hado)) || (rc = a->gacc_off.upload(goff)) ||
-        (rc = a->ckpt.alloc(T * (size_t)(a->nseg + 1))) || (rc = a->stg.alloc(T * 4 * (size_t)(a->C + 1))) ||
-        (rc = a->acc_f.alloc(T)) || (rc = a->lam.alloc(T)) || (rc = a->aacc.alloc(T)) || (rc = a->q1.alloc(T)) ||
-        (rc = a->q2.alloc(T)) || (rc = a->VH.alloc(im->V.n)) || (rc = a->gP.alloc(im->P.n)) ||
-        (rc = a->gacc.alloc((size_t)gtot)))
+        (a->trajm ? (rc = a->traj.alloc(T * (4 * (size_t)b->max_steps + 1)))
+                  : ((rc = a->ckpt.alloc(T * (size_t)(a->nseg + 1))) || (rc = a->stg.alloc(T * 4 * (size_t)(a->C + 1))) ||
+                     (rc = a->acc_f.alloc(T)))) ||
+        (rc = a->lam.alloc(T)) || (rc = a->aacc.alloc(T)) || (rc = a->q1.alloc(T)) || (rc = a->q2.alloc(T)) ||
+        (rc = a->VH.alloc(im->V.n)) || (rc = a->gP.alloc(im->P.n)) || (rc = a->gacc.alloc((size_t)gtot)))
         return rc;
     const int blocks = (int)std::min<long long>(std::max<long long>(1, ((long long)T + 255) / 256), 65536);
     hipStream_t s = im->stream;
     StageArgs fw = stage_args(im);
     PrologueArgs pa = prologue_args(im);
 
-    // forward: C steps + checkpoint store + tick
+    // trajectory slot st of the current step (the kernels add step * 4T)
+    auto tslot = [&](int st) { return a->traj.p + (size_t)st * T; };
+    if (a->trajm) {
+        pa.y = tslot(0);
+        pa.traj = 4 * (long long)T;
+    }
+    // forward: C steps + (checkpoint store) + tick
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < a->C; k++) {
         pa.step_off = k;
         hipLaunchKernelGGL(prologue, dim3(b->n_inst), dim3(256), 0, s, pa);
         for (int st = 0; st < 4; st++) {
-            const StageArgs x = stage_io(fw, im, k, st);
+            StageArgs x = stage_io(fw, im, k, st);
+            if (a->trajm) {   // Y_i, S1, S2, S1' of step i in slots 4i .. 4i+3; Y_{i+1} in 4i+4
+                x.y = tslot(0);
+                x.rin = tslot(st);
+                x.out = tslot(st + 1);
+                x.traj = 4 * (long long)T;
+            }
             for (const Group& g : im->groups)
                 if ((rc = launch_stage(g, x, s))) {
                     hipGraph_t junk;
@@ -555,8 +688,9 @@ int batch_enable_adjoint(Batch* b, int C) {
                 }
         }
     }
-    hipLaunchKernelGGL(ckpt_store, dim3(blocks), dim3(256), 0, s, im->Y.p, a->ckpt.p, (long long)T, im->step_ctr.p,
-                       a->C, a->nseg + 1);
+    if (!a->trajm)
+        hipLaunchKernelGGL(ckpt_store, dim3(blocks), dim3(256), 0, s, im->Y.p, a->ckpt.p, (long long)T,
+                           im->step_ctr.p, a->C, a->nseg + 1);
     hipLaunchKernelGGL(tick, dim3(1), dim3(1), 0, s, im->step_ctr.p, a->C);
     HIPCHK(hipStreamEndCapture(s, &a->gfwd));
     HIPCHK(hipGraphInstantiate(&a->xfwd, a->gfwd, nullptr, nullptr, 0));
@@ -583,6 +717,7 @@ int batch_enable_adjoint(Batch* b, int C) {
     aj.field = im->field.p;
     aj.gacc = a->gacc.p;
     aj.gacc_off = a->gacc_off.p;
+    aj.ncopy = a->ncopy;
     AdjPrologueArgs ap;
     ap.inst = im->inst.p;
     ap.step_ctr = im->step_ctr.p;
@@ -597,10 +732,14 @@ int batch_enable_adjoint(Batch* b, int C) {
     ar.lam = a->lam.p;
     ar.gacc = a->gacc.p;
     ar.gacc_off = a->gacc_off.p;
+    ar.traj = a->trajm ? 4 * (long long)T : 0;
+    aj.traj = a->trajm ? 4 * (long long)T : 0;
     pa.readout = 0;   // replays must not overwrite the forward readouts
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    hipLaunchKernelGGL(ckpt_load, dim3(blocks), dim3(256), 0, s, a->ckpt.p, stg(0, 0), (long long)T, im->step_ctr.p, a->C);
-    for (int k = 0; k < a->C; k++) {
+    if (!a->trajm)
+        hipLaunchKernelGGL(ckpt_load, dim3(blocks), dim3(256), 0, s, a->ckpt.p, stg(0, 0), (long long)T,
+                           im->step_ctr.p, a->C);
+    for (int k = 0; k < a->C && !a->trajm; k++) {
         pa.step_off = k;
         hipLaunchKernelGGL(prologue, dim3(b->n_inst), dim3(256), 0, s, pa);
         for (int st = 0; st < 4; st++) {
@@ -632,7 +771,7 @@ int batch_enable_adjoint(Batch* b, int C) {
             x.S.acc = a->aacc.p;
             x.S.rin = st == 0 ? a->lam.p : (st == 2 ? a->q2.p : a->q1.p);
             x.S.out = st == 3 ? a->lam.p : (st == 1 ? a->q2.p : a->q1.p);
-            x.Yf = stg(k, 3 - st);
+            x.Yf = a->trajm ? tslot(3 - st) : stg(k, 3 - st);
             for (const Group& g : im->groups)
                 if ((rc = launch_adj(g, x, s))) {
                     hipGraph_t junk;
@@ -641,7 +780,7 @@ int batch_enable_adjoint(Batch* b, int C) {
                 }
         }
         ar.step_off = k;
-        ar.y = stg(k, 0);
+        ar.y = a->trajm ? tslot(0) : stg(k, 0);
         hipLaunchKernelGGL(adj_readout, dim3(b->n_inst), dim3(256), 0, s, ar);
     }
     hipLaunchKernelGGL(tick, dim3(1), dim3(1), 0, s, im->step_ctr.p, -a->C);
@@ -655,8 +794,14 @@ int batch_enable_adjoint(Batch* b, int C) {
 int adj_forward(Batch* b) {
     BatchImpl* im = b->impl;
     AdjState* a = im->adj;
-    HIPCHK(hipMemcpyAsync(a->ckpt.p, im->Y.p, (size_t)a->total * sizeof(double2), hipMemcpyDeviceToDevice, im->stream));
+    double2* first = a->trajm ? a->traj.p : a->ckpt.p;
+    HIPCHK(hipMemcpyAsync(first, im->Y.p, (size_t)a->total * sizeof(double2), hipMemcpyDeviceToDevice, im->stream));
     for (int c = 0; c < a->nseg; c++) HIPCHK(hipGraphLaunch(a->xfwd, im->stream));
+    if (a->trajm) {   // final states back into Y (each instance ended at its own step count)
+        hipLaunchKernelGGL(traj_final, dim3(b->n_inst), dim3(256), 0, im->stream, a->traj.p, im->Y.p, im->inst.p,
+                           a->total);
+        HIPCHK(hipGetLastError());
+    }
     return EH_OK;
 }
 
@@ -680,6 +825,16 @@ int batch_backward(Batch* b, const double* dP) {
     HIPCHK(hipStreamSynchronize(s));
     a->h_gacc.resize(a->gacc.n);
     HIPCHK(hipMemcpy(a->h_gacc.data(), a->gacc.p, a->gacc.n * sizeof(double2), hipMemcpyDeviceToHost));
+    for (int i = 0; i < b->n_inst; i++) {   // fold the copies into the first
+        const InstDev& I = im->hinst[(size_t)i];
+        const long long gs = gacc_size(I.N * I.N, I.d);
+        double2* blk = a->h_gacc.data() + a->h_gacc_off[(size_t)i];
+        for (int c = 1; c < a->ncopy; c++)
+            for (long long e = 0; e < gs; e++) {
+                blk[e].x += blk[c * gs + e].x;
+                blk[e].y += blk[c * gs + e].y;
+            }
+    }
     a->have_grad = true;
     return EH_OK;
 }

@@ -56,7 +56,20 @@ struct StageArgs {
     // columns are gathered from rin directly.
     const double2* cin;
     double2* cout;
+    // Trajectory mode: rin, out and y are slots of a per-step array and move by
+    // step * traj elements (0: fixed buffers)
+    long long traj;
 };
+
+// the stage's buffers at this step (trajectory mode)
+__device__ __forceinline__ void at_step(StageArgs& A, int step) {
+    if (A.traj) {
+        const long long o = (long long)step * A.traj;
+        A.rin += o;
+        A.out += o;
+        A.y += o;
+    }
+}
 
 __device__ __forceinline__ double2 cmad(double2 acc, double2 a, double2 b) {
     acc.x = fma(a.x, b.x, fma(-a.y, b.y, acc.x));
@@ -303,6 +316,7 @@ struct PrologueArgs {
     double2* V;
     double2* P;
     int readout;             // 0: build V only (replays in the backward pass)
+    long long traj;          // trajectory mode: y moves by step * traj elements
 };
 
 // V(t_i) = H - e_1 X[0] - e_2 X[1]^T - e_3 X[2]^T  (dynamics.cpp:348-363), and
@@ -331,7 +345,7 @@ static __global__ __launch_bounds__(256) void prologue(PrologueArgs A) {
     const int slot = A.pidx[I.step_off + step];
     if (slot < 0 || !A.readout) return;
     __shared__ double2 red[256];
-    const double2* rho0 = A.y + I.state_off;
+    const double2* rho0 = A.y + (long long)step * A.traj + I.state_off;
     for (int r = 0; r < I.nread; ++r) {
         const double2* R = A.R + I.r_off + (size_t)r * NN;
         double2 s = make_double2(0.0, 0.0);

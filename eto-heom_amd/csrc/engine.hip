@@ -57,6 +57,7 @@ __global__ __launch_bounds__(256) void stage_small(StageArgs A) {
     const InstDev I = A.inst[A.tile_inst[tile]];
     const int step = *A.step_ctr + A.step_off;
     if (step >= I.nsteps) return;
+    at_step(A, step);
     const int a0 = A.tile_ado0[tile], na = A.tile_nado[tile];
     const long long base = I.state_off + (long long)a0 * NN;
     const int nel = na * NN;
@@ -154,6 +155,7 @@ __global__ __launch_bounds__(256) void stage_generic(StageArgs A) {
     const InstDev I = A.inst[A.tile_inst[tile]];
     const int step = *A.step_ctr + A.step_off;
     if (step >= I.nsteps) return;
+    at_step(A, step);
     const int N = I.N, NN = N * N;
     double2* sR = smem;
     double2* sC = smem + NN;
@@ -226,6 +228,7 @@ __global__ __launch_bounds__(256, NP == 32 ? 4 : 2) void stage_mfma(StageArgs A)
     InstT I = A.inst[A.tile_inst[tile]];
     const int step = *A.step_ctr + A.step_off;
     if (step >= I.nsteps) return;
+    at_step(A, step);
     const int N = I.N, NN = N * N;
     const int a0 = A.tile_ado0[tile];
     const long long base = I.state_off + (long long)a0 * NN;
@@ -476,6 +479,7 @@ __global__ __launch_bounds__(256, 2) void stage_mfma_pipe(StageArgs A, int ntile
     double2* const sC0 = reinterpret_cast<double2*>(sVi + NP * LD);
     double2* const sC1 = sC0 + cpn;
     const int step = *A.step_ctr + A.step_off;
+    at_step(A, step);
     int t = blockIdx.x;
     if (t >= ntiles) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -677,7 +681,7 @@ static inline double2 d2(cplx c) { return make_double2(c.real(), c.imag()); }
 
 int launch_cfg(const Group& g, const LaunchCfg& c, const StageArgs& base, hipStream_t s) {
     StageArgs a = base;
-    if (!c.cols) {   // this configuration gathers columns from rin
+    if (!c.cols || a.traj) {   // this configuration (or trajectory mode) gathers columns from rin
         a.cin = nullptr;
         a.cout = nullptr;
     }

@@ -80,8 +80,12 @@ def small_dimer():
     return p
 
 
-@pytest.mark.parametrize("C", [160, 7])
-def test_adjoint_matches_oracle_fd_dimer(ctx, C):
+@pytest.mark.parametrize("C,mode", [(160, "trajectory"), (7, "trajectory"), (7, "replay")])
+def test_adjoint_matches_oracle_fd_dimer(ctx, C, mode, monkeypatch):
+    """Both backward modes: every stage input kept from the forward (trajectory), or
+    checkpoints every C steps with a replay per segment."""
+    if mode == "replay":
+        monkeypatch.setenv("ETOHEOM_ADJ_REPLAY", "1")
     p = small_dimer()
     P0, _ = O.twod(oproblem(p), nv=(1, 1, 1))
     rng = np.random.default_rng(1)
