@@ -158,7 +158,8 @@ int eh_batch_dims(const eh_batch* b, int64_t* out5);
 int eh_batch_profile(eh_batch* b, int nsteps, double* out5);
 /* the plain per-launch path: same kernels and results as eh_batch_run, launched one
  * by one from the host step after step, no graph (the reference GPU path's
- * per-propagation launch pattern, etoheom-gpu/src/HEOM_dynamics.cu:494-560); for
+ * per-propagation launch pattern, etoheom-gpu/lib/HEOM_dynamics.cu:489-658, time
+ * loop :566); for
  * comparison with the graph-replayed batch; asynchronous like eh_batch_run */
 int eh_batch_run_eager(eh_batch* b);
 /* autotuner: time `steps` RK4 steps of every launch candidate of every N class
