@@ -218,7 +218,7 @@ __global__ __launch_bounds__(256) void adj_small(AdjArgs J) {
     const int total = 2 * I.K * NT * na;
     const bool has_cpl = total > 0 && I.Km > 0;
     NbIdx<2, 2> nbx;
-    if (has_cpl) gather_idx<2, 2>(A, I, a0, na, threadIdx.x, 256, nbx);
+    if (has_cpl) gather_idx<2, 2, NT>(A, I, a0, na, threadIdx.x, 256, nbx);
     const bool vreal = A.vreal[I.step_off + step] != 0;
     const double2* lp = (st == 1 || st == 2) ? A.y : A.rin;
     const double2* ap = st > 0 ? A.acc : A.rin;
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(256) void adj_small(AdjArgs J) {
         vv[q] = A.V[I.v_off + (e < NN ? e : 0)];
     }
     if (has_cpl) {
-        gather_data<2, 2>(A, I, na, threadIdx.x, 256, nbx, sC);
+        gather_data<2, 2, NT>(A, I, na, threadIdx.x, 256, nbx, sC);
         if (total > 2 * 256 || I.Km > 2) gather_rest<2, 2>(A, I, a0, na, threadIdx.x, 256, sC);
     }
     for (int e = threadIdx.x; e < 6 * I.d; e += 256) sU[e] = 0.0;
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(256) void adj_small(AdjArgs J) {
         const double2 g = gv[q], r = R[ab];
         ur = fma(g.x, r.x, fma(-g.y, r.y, ur));
         ui = fma(g.x, r.y, fma(g.y, r.x, ui));
-        const double2 cpl = coupling_of(I, sC + nl * cpn, ra, cb);
+        const double2 cpl = coupling_of<NT>(I, sC + nl * cpn, ra, cb);
         const double2 lam = st == 0 ? r : lv[q];
         rk_store_adj(A, I, (size_t)(base + e), lam, st == 0 ? make_double2(0.0, 0.0) : av[q], ur + cpl.x,
                      ui + cpl.y);

@@ -25,6 +25,7 @@ struct InstDev {
     int al_off;            // offset of alpha / alpha~ (K*Km complex)
     double h2, h, hb[4];
     long long col_off;     // complex offset of ADO 0 in the coupled-column copies (K*N per ADO)
+    long long vp_off;      // offset of the zero-padded 64x64 real/imag planes of V (N > 32), else -1
 };
 
 struct StageArgs {
@@ -37,6 +38,7 @@ struct StageArgs {
     int stage;
     const uint8_t* vreal;    // per step: V(t_i) has no imaginary part
     const double2* V;        // per instance N*N
+    const double* Vpad;      // per instance with N > 32: zero-padded 64x64 real, imag planes
     const int* nb_up;        // per (ADO, p): index of n + e_p or -1
     const int* nb_dn;        // per (ADO, p): index of n - e_p or -1
     const double* cx;        // per (ADO, p): sqrt((n_p + 1) |alpha_p|)
@@ -107,11 +109,14 @@ __device__ __forceinline__ void rk_store(const StageArgs& A, const InstDev& I, s
 
 // address of element (a, b) = (pos, s) [rc = 1] or (s, pos) [rc = 0] of neighbour mu
 // for bath j: columns from the column copy when the stage has one
+template <int NT = 0>
 __device__ __forceinline__ const double2* nb_elem(const StageArgs& A, const InstDev& I, int mu, int j, int rc,
                                                   int pos) {
-    const int N = I.N, s = j + I.so;
-    if (rc == 1 && A.cin) return A.cin + I.col_off + ((long long)mu * I.K + j) * N + pos;
-    return A.rin + I.state_off + (long long)mu * N * N + (rc == 0 ? s * N + pos : pos * N + s);
+    // 32-bit offsets inside the instance (an instance holds < 2^32 elements), one 64-bit add
+    const int N = NT > 0 ? NT : I.N, s = j + I.so;
+    // the small-N kernels (NT > 0) keep no column copies (measured: no gain below N = 16)
+    if (NT == 0 && rc == 1 && A.cin) return (A.cin + I.col_off) + (unsigned)((mu * I.K + j) * N + pos);
+    return (A.rin + I.state_off) + (unsigned)(mu * N * N + (rc == 0 ? s * N + pos : pos * N + s));
 }
 
 // y and acc of element gi as the stage needs them (stage 0 takes y = rin value)
@@ -181,23 +186,39 @@ struct NbIdx {
     double2 al[EC][KMX];
 };
 
-template <int EC, int KMX>
+// entry e of a tile's coupling rows/columns, e = ((g K + j) 2 + rc) N + pos, split into
+// (ADO g, bath j, row/column rc, position pos).  NT > 0: N is a compile-time constant;
+// the division by the runtime K goes through a float reciprocal (exact here: the
+// quotient is at least 0.5/K from an integer and g K + j < 2^16).
+template <int NT>
+__device__ __forceinline__ void entry_split(int e, int N, int K, float invK, int& g, int& j, int& rc, int& pos) {
+    const int n = NT > 0 ? NT : N;
+    const int q = e / n;
+    pos = e - q * n;
+    rc = q & 1;
+    const int gj = q >> 1;
+    g = (int)(((float)gj + 0.5f) * invK);
+    j = gj - g * K;
+}
+
+template <int EC, int KMX, int NT = 0>
 __device__ __forceinline__ void gather_idx(const StageArgs& A, const InstDev& I, int n0, int na, int tid, int nthr,
                                            NbIdx<EC, KMX>& X) {
     // raw values only: masking them here would make the thread wait for each load
-    const int N = I.N, K = I.K, Km = I.Km, d = I.d;
+    const int N = NT > 0 ? NT : I.N, K = I.K, Km = I.Km, d = I.d;
     const int per = 2 * K * N, total = per * na;
+    const float invK = 1.0f / (float)K;
 #pragma unroll
     for (int r = 0; r < EC; ++r) {
         const int e = tid + r * nthr;
         const int ec = e < total ? e : total - 1;
-        const int g = ec / per, eg = ec - g * per;
-        const int j = eg / (2 * N), rc = (eg - j * 2 * N) / N;
-        const long long q0 = I.lk_off + (long long)(n0 + g) * d + j;
+        int g, j, rc, pos;
+        entry_split<NT>(ec, N, K, invK, g, j, rc, pos);
+        const unsigned q0 = (unsigned)((n0 + g) * d + j);   // row in the instance's tables
 #pragma unroll
         for (int k = 0; k < KMX; ++k) {
             const int kc = k < Km ? k : Km - 1;
-            const long long q = q0 + (long long)kc * K;
+            const long long q = I.lk_off + (q0 + (unsigned)(kc * K));
             X.up[r][k] = A.nb_up[q];
             X.dn[r][k] = A.nb_dn[q];
             X.x[r][k] = A.cx[q];
@@ -207,26 +228,26 @@ __device__ __forceinline__ void gather_idx(const StageArgs& A, const InstDev& I,
     }
 }
 
-template <int EC, int KMX>
+template <int EC, int KMX, int NT = 0>
 __device__ __forceinline__ void gather_data(const StageArgs& A, const InstDev& I, int na, int tid, int nthr,
                                             const NbIdx<EC, KMX>& X, double2* cp) {
-    const int N = I.N, K = I.K, Km = I.Km;
+    const int N = NT > 0 ? NT : I.N, K = I.K, Km = I.Km;
     const int per = 2 * K * N, total = per * na;
+    const float invK = 1.0f / (float)K;
     double2 xu[EC][KMX], xd[EC][KMX];
     int rcs[EC];
 #pragma unroll
     for (int r = 0; r < EC; ++r) {
         const int e = tid + r * nthr;
         const int ec = e < total ? e : total - 1;
-        const int eg = ec - (ec / per) * per;
-        const int j = eg / (2 * N), rem = eg - j * 2 * N;
-        const int rc = rem / N, pos = rem - rc * N;
+        int g, j, rc, pos;
+        entry_split<NT>(ec, N, K, invK, g, j, rc, pos);
         rcs[r] = rc;
 #pragma unroll
         for (int k = 0; k < KMX; ++k) {
             const int mu = X.up[r][k], md = X.dn[r][k];
-            xu[r][k] = *nb_elem(A, I, mu > 0 ? mu : 0, j, rc, pos);
-            xd[r][k] = *nb_elem(A, I, md > 0 ? md : 0, j, rc, pos);
+            xu[r][k] = *nb_elem<NT>(A, I, mu > 0 ? mu : 0, j, rc, pos);
+            xd[r][k] = *nb_elem<NT>(A, I, md > 0 ? md : 0, j, rc, pos);
         }
     }
 #pragma unroll
@@ -280,16 +301,18 @@ __device__ __forceinline__ void gather_rest(const StageArgs& A, const InstDev& I
 }
 
 // coupling term of element (row, col) from the gathered rows/columns
+template <int NT = 0>
 __device__ __forceinline__ double2 coupling_of(const InstDev& I, const double2* cp, int row, int col) {
+    const int N = NT > 0 ? NT : I.N;
     double2 c = make_double2(0.0, 0.0);
     const int jr = row - I.so, jc = col - I.so;
     if (jr >= 0 && jr < I.K) {
-        const double2 v = cp[(jr * 2 + 0) * I.N + col];
+        const double2 v = cp[(jr * 2 + 0) * N + col];
         c.x += v.x;
         c.y += v.y;
     }
     if (jc >= 0 && jc < I.K) {
-        const double2 v = cp[(jc * 2 + 1) * I.N + row];
+        const double2 v = cp[(jc * 2 + 1) * N + row];
         c.x += v.x;
         c.y += v.y;
     }
@@ -314,6 +337,7 @@ struct PrologueArgs {
     const double2* R;        // per instance nread*N*N readout operators
     const double2* y;
     double2* V;
+    double* Vpad;            // zero-padded 64x64 real and imaginary planes of V (instances with N > 32)
     double2* P;
     int readout;             // 0: build V only (replays in the backward pass)
     long long traj;          // trajectory mode: y moves by step * traj elements
@@ -341,6 +365,27 @@ static __global__ __launch_bounds__(256) void prologue(PrologueArgs A) {
         v.x -= x2.x * e2.x - x2.y * e2.y;
         v.y -= x2.x * e2.y + x2.y * e2.x;
         V[q] = v;
+    }
+    if (I.vp_off >= 0) {   // the same V, zero-padded to 64 x 64 as real and imaginary planes
+        double* vr = A.Vpad + I.vp_off;
+        double* vi = vr + 64 * 64;
+        for (int q = threadIdx.x; q < 64 * 64; q += blockDim.x) {
+            const int r = q >> 6, c = q & 63;
+            double2 v = make_double2(0.0, 0.0);
+            if (r < N && c < N) {
+                const int ix = r * N + c;
+                v = H[ix];
+                const double2 x0 = X[ix], x1 = X[NN + ix], x2 = X[2 * NN + ix];
+                v.x -= x0.x * e0.x - x0.y * e0.y;
+                v.y -= x0.x * e0.y + x0.y * e0.x;
+                v.x -= x1.x * e1.x - x1.y * e1.y;
+                v.y -= x1.x * e1.y + x1.y * e1.x;
+                v.x -= x2.x * e2.x - x2.y * e2.y;
+                v.y -= x2.x * e2.y + x2.y * e2.x;
+            }
+            vr[q] = v.x;
+            vi[q] = v.y;
+        }
     }
     const int slot = A.pidx[I.step_off + step];
     if (slot < 0 || !A.readout) return;

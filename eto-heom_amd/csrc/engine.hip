@@ -67,7 +67,7 @@ __global__ __launch_bounds__(256) void stage_small(StageArgs A) {
     const int total = 2 * I.K * NT * na;
     const bool has_cpl = total > 0 && I.Km > 0;
     NbIdx<2, 2> nbx;
-    if (has_cpl) gather_idx<2, 2>(A, I, a0, na, threadIdx.x, 256, nbx);
+    if (has_cpl) gather_idx<2, 2, NT>(A, I, a0, na, threadIdx.x, 256, nbx);
     const bool vreal = A.vreal[I.step_off + step] != 0;
     // y and acc where the stage reads them, else the (just loaded, cached) input
     const int st = A.stage;
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void stage_small(StageArgs A) {
     }
     const int cpn = 2 * I.K * NT;
     if (has_cpl) {
-        gather_data<2, 2>(A, I, na, threadIdx.x, 256, nbx, sC);
+        gather_data<2, 2, NT>(A, I, na, threadIdx.x, 256, nbx, sC);
         if (total > 2 * 256 || I.Km > 2) gather_rest<2, 2>(A, I, a0, na, threadIdx.x, 256, sC);
     }
 #pragma unroll
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(256) void stage_small(StageArgs A) {
         const double2 g = gv[q], r = rv[q];
         dr = fma(g.x, r.x, fma(-g.y, r.y, dr));
         di = fma(g.x, r.y, fma(g.y, r.x, di));
-        const double2 cpl = coupling_of(I, sC + nl * cpn, ra, cb);
+        const double2 cpl = coupling_of<NT>(I, sC + nl * cpn, ra, cb);
         dr += cpl.x;
         di += cpl.y;
         rk_store(A, I, (size_t)(base + e), st == 0 ? r : yv[q], st == 0 ? make_double2(0.0, 0.0) : av[q], dr, di,
@@ -327,6 +327,139 @@ __global__ __launch_bounds__(256, NP == 32 ? 4 : 2) void stage_mfma(StageArgs A)
                 const double2 cpl = coupling_of(I, sC, row, col);
                 const double2 y0 = A.stage == 0 ? x : yv[t][r];
                 rk_store(A, I, (size_t)(base + row * N + col), y0, av[t][r], dr + cpl.x, di + cpl.y, a0, row, col);
+            }
+        }
+    }
+}
+
+// rho_n[r][c] from its N x (N+1) LDS copy, 0 in the padding up to 64 (clamped read, selected)
+__device__ __forceinline__ double2 lds_at(const double2* sR, int r, int c, int N, int LD) {
+    const bool in = r < N && c < N;
+    const double2 v = sR[(in ? r : 0) * LD + (in ? c : 0)];
+    return in ? v : make_double2(0.0, 0.0);
+}
+
+// N in (32, 64]: one ADO per workgroup of 512 threads (8 waves, each two 16x16
+// output tiles of one tile row), two workgroups per CU so that one computes while
+// the other loads.  rho_n sits in LDS (complex, stride 65); V comes from the
+// instance's zero-padded 64x64 real / imaginary planes in global memory (shared by
+// all the instance's workgroups, cache resident; the loads are unconditional); y and
+// acc are loaded after the commutator to keep the registers of two workgroups.
+__global__ __launch_bounds__(512, 2) void stage_mfma64(StageArgs A) {
+    constexpr int NP = 64, SPT = NP * NP / 512;
+    extern __shared__ double2 smem[];
+    const int tile = blockIdx.x;
+    const InstDev& I = A.inst[A.tile_inst[tile]];
+    const int step = *A.step_ctr + A.step_off;
+    if (step >= I.nsteps) return;
+    at_step(A, step);
+    const int N = I.N, NN = N * N;
+    const int LD = N + 1;   // rho_n in LDS at its own size (N x (N+1)): two workgroups fit a CU
+    double2* sR = smem;
+    double2* sC = smem + N * LD;
+    const int a0 = A.tile_ado0[tile];
+    const long long base = I.state_off + (long long)a0 * NN;
+    const int st = A.stage;
+    const bool vreal = A.vreal[I.step_off + step] != 0;
+    const double2 g = A.damp[I.ado_off + a0];
+    const double* Vr = A.Vpad + I.vp_off;
+    const double* Vi = Vr + NP * NP;
+    double2 xr[SPT];
+#pragma unroll
+    for (int q = 0; q < SPT; ++q) {
+        const int e = threadIdx.x + q * 512;
+        const int r = e >> 6, c = e & 63;
+        const bool in = r < N && c < N;
+        const double2 t = A.rin[base + (in ? r * N + c : 0)];
+        xr[q] = in ? t : make_double2(0.0, 0.0);
+    }
+    gather_couplings(A, I, a0, 1, sC, threadIdx.x, 512);
+#pragma unroll
+    for (int q = 0; q < SPT; ++q) {
+        const int e = threadIdx.x + q * 512;
+        const int r = e >> 6, c = e & 63;
+        if (r < N && c < N) sR[r * LD + c] = xr[q];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int ti = wave >> 1, tj0 = (wave & 1) * 2;
+    const int ar = ti * 16 + li;
+    dbl4 cr[2], ci[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        cr[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+        ci[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+    }
+    if (vreal) {
+#pragma unroll 4
+        for (int k0 = 0; k0 < NP; k0 += 4) {
+            const int kk = k0 + lk;
+            const double va = Vr[ar * NP + kk];
+            const double2 ra = lds_at(sR, ar, kk, N, LD);
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int bc = (tj0 + t) * 16 + li;
+                const double vb = Vr[kk * NP + bc];
+                const double2 rb = lds_at(sR, kk, bc, N, LD);
+                cr[t] = mfma4(va, rb.x, cr[t]);
+                ci[t] = mfma4(va, rb.y, ci[t]);
+                cr[t] = mfma4(ra.x, -vb, cr[t]);
+                ci[t] = mfma4(ra.y, -vb, ci[t]);
+            }
+        }
+    } else {
+#pragma unroll 2
+        for (int k0 = 0; k0 < NP; k0 += 4) {
+            const int kk = k0 + lk;
+            const double var = Vr[ar * NP + kk], vai = Vi[ar * NP + kk];
+            const double2 ra = lds_at(sR, ar, kk, N, LD);
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int bc = (tj0 + t) * 16 + li;
+                const double vbr = Vr[kk * NP + bc], vbi = Vi[kk * NP + bc];
+                const double2 rb = lds_at(sR, kk, bc, N, LD);
+                cr[t] = mfma4(var, rb.x, cr[t]);
+                ci[t] = mfma4(var, rb.y, ci[t]);
+                cr[t] = mfma4(-vai, rb.y, cr[t]);
+                ci[t] = mfma4(vai, rb.x, ci[t]);
+                cr[t] = mfma4(ra.x, -vbr, cr[t]);
+                ci[t] = mfma4(ra.y, -vbr, ci[t]);
+                cr[t] = mfma4(ra.y, vbi, cr[t]);
+                ci[t] = mfma4(ra.x, -vbi, ci[t]);
+            }
+        }
+    }
+    // y and acc of the lane's 8 output elements (unconditional, clamped), then the update
+    const double2* yp = (st == 1 || st == 2) ? A.y : A.rin;
+    const double2* ap = st != 0 ? A.acc : A.rin;
+    double2 yv[2][4], av[2][4];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int col = (tj0 + t) * 16 + li;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = ti * 16 + lk + 4 * r;
+            const bool in = row < N && col < N;
+            const size_t gi = (size_t)(base + (in ? row * N + col : 0));
+            yv[t][r] = yp[gi];
+            av[t][r] = ap[gi];
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int col = (tj0 + t) * 16 + li;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = ti * 16 + lk + 4 * r;
+            if (row < N && col < N) {
+                double dr = ci[t][r], di = -cr[t][r];   // -i * C
+                const double2 x = sR[row * LD + col];
+                dr = fma(g.x, x.x, fma(-g.y, x.y, dr));
+                di = fma(g.x, x.y, fma(g.y, x.x, di));
+                const double2 cpl = coupling_of(I, sC, row, col);
+                rk_store(A, I, (size_t)(base + row * N + col), st == 0 ? x : yv[t][r],
+                         st == 0 ? make_double2(0.0, 0.0) : av[t][r], dr + cpl.x, di + cpl.y, a0, row, col);
             }
         }
     }
@@ -704,6 +837,7 @@ int launch_cfg(const Group& g, const LaunchCfg& c, const StageArgs& base, hipStr
 #undef EH_CASE
         case -32: hipLaunchKernelGGL((stage_mfma<32, true>), grid, block, c.shmem, s, a); break;
         case -64: hipLaunchKernelGGL((stage_mfma<64, false>), grid, block, c.shmem, s, a); break;
+        case -65: hipLaunchKernelGGL(stage_mfma64, grid, dim3(512), c.shmem, s, a); break;
         case -33:
             // the RK coefficients of this stage are arguments (one time step per group)
             hipLaunchKernelGGL(stage_mfma_pipe<2>, dim3(std::min(c.tile_count, g.persist)), block, c.shmem, s, a,
@@ -736,6 +870,7 @@ StageArgs stage_args(const BatchImpl* im) {
     sa.step_ctr = im->step_ctr.p;
     sa.vreal = im->vreal.p;
     sa.V = im->V.p;
+    sa.Vpad = im->Vpad.p;
     sa.nb_up = im->nb_up.p;
     sa.nb_dn = im->nb_dn.p;
     sa.cx = im->cx.p;
@@ -761,6 +896,7 @@ PrologueArgs prologue_args(const BatchImpl* im) {
     pa.R = im->R.p;
     pa.y = im->Y.p;
     pa.V = im->V.p;
+    pa.Vpad = im->Vpad.p;
     pa.P = im->P.p;
     pa.readout = 1;
     return pa;
@@ -837,7 +973,7 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
     std::vector<uint8_t> hvreal;
     std::vector<double2> hH, hXp, hfield, hR, hal, halt, hdamp;
     std::vector<long long> hado0;
-    int64_t state_off = 0, p_off = 0, lk_off = 0, col_off = 0;
+    int64_t state_off = 0, p_off = 0, lk_off = 0, col_off = 0, vp_off = 0;
     int ado_off = 0, v_off = 0, r_off = 0, step_off = 0, al_off = 0;
     // instances by N class: 1..16 (each N its own class), 17 (16 < N <= 32), 18 (32 < N <= 64)
     std::vector<std::vector<int>> by_class(19);
@@ -930,6 +1066,11 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
         d.al_off = al_off;
         d.col_off = col_off;
         col_off += (int64_t)M * sys.K * N;
+        d.vp_off = -1;
+        if (N > 32) {
+            d.vp_off = vp_off;
+            vp_off += 2 * 64 * 64;
+        }
         d.h2 = s.h2;
         d.h = s.h;
         for (int k = 0; k < 4; k++) d.hb[k] = s.hb[k];
@@ -960,6 +1101,35 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
     // Groups and their launch candidates.  Tiles are (instance, first ADO, ADO count)
     // triples; a group's candidates with equal ADOs per tile share one tile range.
     std::vector<int> hginst;
+    // XCD-aware tile order: workgroups go to the XCDs round-robin (workgroup w to XCD
+    // w mod xcd), each XCD with its own L2, so a range of tiles is laid out such that
+    // XCD x runs the x-th contiguous slice of the (instance, ADO) order: an instance's
+    // neighbour gathers then stay in one L2 instead of all of them.
+    const int xcd = env_on("ETOHEOM_NO_XCD") ? 1 : (ctx->n_cu % 8 == 0 && ctx->n_cu >= 64 ? 8 : 1);
+    auto xcd_order = [&](int begin, int count) {
+        if (count < 2 * xcd) return;
+        std::vector<int> qi(t_inst.begin() + begin, t_inst.begin() + begin + count);
+        std::vector<int> qa(t_ado0.begin() + begin, t_ado0.begin() + begin + count);
+        std::vector<int> qn(t_nado.begin() + begin, t_nado.begin() + begin + count);
+        std::vector<int> head(xcd), end(xcd);
+        for (int x = 0; x < xcd; x++) {
+            head[x] = (int)((long long)count * x / xcd);
+            end[x] = (int)((long long)count * (x + 1) / xcd);
+        }
+        for (int p = 0; p < count; p++) {
+            int x = p % xcd;
+            if (head[x] == end[x]) {   // this slice is done: take from the fullest one
+                int best = 0;
+                for (int y = 1; y < xcd; y++)
+                    if (end[y] - head[y] > end[best] - head[best]) best = y;
+                x = best;
+            }
+            const int src = head[x]++;
+            t_inst[(size_t)(begin + p)] = qi[(size_t)src];
+            t_ado0[(size_t)(begin + p)] = qa[(size_t)src];
+            t_nado[(size_t)(begin + p)] = qn[(size_t)src];
+        }
+    };
     for (int cls = 1; cls <= 18; cls++) {
         const auto& members = by_class[(size_t)cls];
         if (members.empty()) continue;
@@ -999,6 +1169,7 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
                 }
             }
             const std::pair<int, int> r(begin, (int)t_inst.size() - begin);
+            if (xcd > 1) xcd_order(begin, r.second);
             ranges.push_back({G, r});
             return r;
         };
@@ -1018,31 +1189,44 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
                 c.shmem = (size_t)2 * 32 * 33 * sizeof(double2) + cpb;
             else if (nt == -64)
                 c.shmem = (size_t)64 * 65 * sizeof(double2) + cpb;
+            else if (nt == -65) {
+                int mn = 1;
+                while (mn * mn < maxnn) mn++;
+                c.shmem = (size_t)mn * (mn + 1) * sizeof(double2) + cpb;
+            }
             else if (nt > 0)
                 c.shmem = (size_t)G * cpb;   // the coupling rows/columns of G ADOs (the rest is static)
             else
                 c.shmem = (size_t)maxnn * sizeof(double2) + cpb;
             if ((int)c.shmem > ctx->lds_per_block) return;   // not a candidate on this device
             if (c.shmem > 65536) {
-                const void* fn = nt == -64 ? (const void*)stage_mfma<64, false>
+                const void* fn = nt == -64   ? (const void*)stage_mfma<64, false>
+                                 : nt == -65 ? (const void*)stage_mfma64
                                  : (nt == -33 ? (const void*)stage_mfma_pipe<2> : (const void*)stage_generic);
                 if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.shmem) != hipSuccess)
                     return;
             }
             g.cand.push_back(c);
         };
-        // column copies: measured to pay off for N > 16 (long strided columns), not below
+        // column copies: measured to pay off for N > 16 (long strided columns), not below,
+        // where the small-N kernel has no column-copy path
         if (cls <= 16) {
             add(cls, 512, false);   // the base configuration: the adjoint's small kernels tile as it does
             add(cls, 256, false);
             add(cls, 1024, false);
-            add(cls, 512, true);
             add(0, 0, false);
-        } else {
-            if (use_mfma) add(cls == 17 ? -32 : -64, 0, true);
+        } else if (cls == 17) {
+            if (use_mfma) add(-32, 0, true);
             add(0, 0, true);
-            if (use_mfma) add(cls == 17 ? -32 : -64, 0, false);
+            if (use_mfma) add(-32, 0, false);
             if (pipe_ok) add(-33, 0, false);
+        } else {   // 32 < N <= 64: stage_mfma64 gathers columns directly (measured faster at N = 56)
+            if (use_mfma) add(-65, 0, false);
+            add(0, 0, true);
+            if (use_mfma) {
+                add(-65, 0, true);
+                add(-64, 0, true);
+            }
         }
         if (g.cand.empty() || (cls <= 16 && g.cand[0].nt != cls))
             return fail(EH_ESHAPE, "state tile exceeds LDS");
@@ -1069,7 +1253,7 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
         (rc = im->Y.alloc((size_t)state_off)) || (rc = im->ACC.alloc((size_t)state_off)) ||
         (rc = im->S1.alloc((size_t)state_off)) || (rc = im->S2.alloc((size_t)state_off)) ||
         (rc = im->Yc.alloc((size_t)col_off)) || (rc = im->S1c.alloc((size_t)col_off)) ||
-        (rc = im->S2c.alloc((size_t)col_off)))
+        (rc = im->S2c.alloc((size_t)col_off)) || (rc = im->Vpad.alloc((size_t)vp_off)))
         return rc;
     im->hinst = hinst;
     im->hup = std::move(hup);
@@ -1296,8 +1480,12 @@ std::string batch_tuning_report(const Batch* b) {
         const Group& g = im->groups[gi];
         for (size_t c = 0; c < g.cand.size(); c++) {
             const LaunchCfg& x = g.cand[c];
-            const char* kn = x.nt > 0 ? "stage_small" : (x.nt == -32 ? "stage_mfma32" : (x.nt == -64 ? "stage_mfma64"
-                             : (x.nt == -33 ? "stage_mfma_pipe" : "stage_generic")));
+            const char* kn = x.nt > 0      ? "stage_small"
+                             : x.nt == -32 ? "stage_mfma32"
+                             : x.nt == -64 ? "stage_mfma64_v1"
+                             : x.nt == -65 ? "stage_mfma64"
+                             : x.nt == -33 ? "stage_mfma_pipe"
+                                           : "stage_generic";
             char line[256];
             std::snprintf(line, sizeof(line),
                           "group %zu cand %zu maxN^2=%d instances=%d %s tile=%d tiles=%d cols=%d ms_per_step=%.6f%s\n",

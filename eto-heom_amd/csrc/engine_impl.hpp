@@ -52,7 +52,8 @@ struct DevBuf {
 
 // One way of launching a group's stage: kernel variant and tiling.
 struct LaunchCfg {
-    int nt;                   // 1..16: stage_small<nt>; -32 / -64: stage_mfma; -33: stage_mfma_pipe; 0: stage_generic
+    int nt;                   // 1..16: stage_small<nt>; -32 / -64: stage_mfma; -65: stage_mfma64;
+                              // -33: stage_mfma_pipe; 0: stage_generic
     int ge;                   // stage_small: state elements per tile (256, 512 or 1024)
     bool cols;                // keep coupled-column copies (contiguous column gathers)
     int tile_begin, tile_count;
@@ -87,6 +88,7 @@ struct BatchImpl {
     DevBuf<double> cx, cy;
     DevBuf<double2> V, H, Xp, field, R, P, alpha, alpha_t, damp, Y, ACC, S1, S2;
     DevBuf<double2> Yc, S1c, S2c;        // coupled-column copies of Y, S1, S2
+    DevBuf<double> Vpad;                 // padded V planes of the N > 32 instances
     DevBuf<long long> ado0_off;
     DevBuf<int> ginst;                   // instance indices, grouped
     std::vector<Group> groups;

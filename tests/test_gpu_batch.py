@@ -1,7 +1,7 @@
 """GPU batch engine: the per-group graph, the plain per-launch path, the launch
 candidates and the autotuner, each against the CPU oracle (<= 1e-10 normwise
-relative, FP64) on one batch that holds four N classes (N = 4, 7, 11, 29) and an
-instance with three modes per bath."""
+relative, FP64) on one batch that holds five N classes (N = 4, 7, 11, 29, 37) and
+an instance with three modes per bath."""
 import numpy as np
 import pytest
 
@@ -34,7 +34,8 @@ def mixed_items():
     items = [configs.aggregate(2, 3, nsteps=120, taus=(10.0, 20.0, 30.0)),      # N = 4
              configs.aggregate(3, 3, nsteps=100, taus=(10.0, 25.0, 35.0)),      # N = 7
              configs.aggregate(4, 2, nsteps=90, taus=(20.0, 10.0, 30.0)),       # N = 11
-             configs.heptamer(L=1, nsteps=40)]                                  # N = 29
+             configs.heptamer(L=1, nsteps=40),                                  # N = 29
+             configs.aggregate(8, 1, nsteps=30, taus=(2.0, 6.0, 10.0))]         # N = 37
     items[3].pulses[:, 1] = [2.0, 6.0, 10.0]
     # three modes per bath (Km = 3): the modes beyond the kernels' two-phase gathers
     km3 = configs.aggregate(2, 2, nsteps=110, taus=(10.0, 20.0, 30.0))
@@ -82,7 +83,7 @@ def test_every_launch_candidate_matches_oracle(ctx, refs):
     b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)} for p in items])
     cands = b.candidates()
     kinds = {k for _, _, k, _ in cands}
-    assert {"stage_small", "stage_generic", "stage_mfma32"} <= kinds
+    assert {"stage_small", "stage_generic", "stage_mfma32", "stage_mfma64"} <= kinds
     assert {t for _, _, k, t in cands if k == "stage_small"} == {256, 512, 1024}
     for g, c, kern, tile in cands:
         print(f"group {g} candidate {c}: {kern} tile={tile}", flush=True)
@@ -101,7 +102,7 @@ def test_autotune_picks_one_candidate_per_group_and_stays_correct(ctx, refs):
     print(rep, flush=True)
     lines = rep.splitlines()
     groups = {int(l.split()[1]) for l in lines}
-    assert len(groups) == 4
+    assert len(groups) == 5
     for g in groups:
         sel = [l for l in lines if int(l.split()[1]) == g and l.endswith("*")]
         assert len(sel) == 1
