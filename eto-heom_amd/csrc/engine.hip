@@ -207,8 +207,11 @@ __global__ __launch_bounds__(256) void stage_generic(StageArgs A) {
 // A[i][k] at lane i + 16k, B[k][j] at lane j + 16k, D[lane/16 + 4r][lane%16] in reg r.
 // All global loads of the workgroup (rho_n, V, coupling gathers, and y / acc of
 // the lane's output elements) are issued before the first wait.
-template <int NP, bool VLDS>
-__global__ __launch_bounds__(256, NP == 32 ? 4 : 2) void stage_mfma(StageArgs A) {
+// TP: two-phase gathers (neighbour indices and weights first, every load
+// unconditional) -- a launch candidate offered when every instance of the group has
+// 2 K N <= 512 and K_m <= 2.
+template <int NP, bool VLDS, bool TP = false>
+__global__ __launch_bounds__(256, NP == 32 ? (TP ? 3 : 4) : 2) void stage_mfma(StageArgs A) {
     constexpr int LD = NP + 1;
     constexpr int NB = NP / 16;
     constexpr int TPW = NB * NB / 4;
@@ -236,34 +239,67 @@ __global__ __launch_bounds__(256, NP == 32 ? 4 : 2) void stage_mfma(StageArgs A)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
 
+    NbIdx<2, 2> nbx;
+    if (TP) gather_idx<2, 2>(A, I, a0, 1, threadIdx.x, 256, nbx);
     double2 xr[SPT], xv[SPT];
-#pragma unroll
-    for (int q = 0; q < SPT; ++q) {
-        const int e = threadIdx.x + q * 256;
-        const int r = e / NP, c = e - r * NP;
-        const bool in = r < N && c < N;
-        xr[q] = in ? A.rin[base + r * N + c] : make_double2(0.0, 0.0);
-        if (VLDS) xv[q] = in ? V[r * N + c] : make_double2(0.0, 0.0);
-    }
     // y / acc of this lane's output elements
     double2 yv[TPW][4], av[TPW][4];
+    if (TP) {   // every load unconditional: clamped address, selected result
+        const double2* yp = (A.stage == 1 || A.stage == 2) ? A.y : A.rin;
+        const double2* ap = A.stage != 0 ? A.acc : A.rin;
 #pragma unroll
-    for (int t = 0; t < TPW; ++t) {
-        const int tt = wave * TPW + t, ti = tt / NB, tj = tt - (tt / NB) * NB;
-        const int col = tj * 16 + li;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = ti * 16 + lk + 4 * r;
-            yv[t][r] = make_double2(0.0, 0.0);
-            av[t][r] = make_double2(0.0, 0.0);
-            if (row < N && col < N && A.stage != 0) {
-                const size_t gi = (size_t)(base + row * N + col);
-                if (A.stage < 3) yv[t][r] = A.y[gi];
-                av[t][r] = A.acc[gi];
+        for (int q = 0; q < SPT; ++q) {
+            const int e = threadIdx.x + q * 256;
+            const int r = e / NP, c = e - r * NP;
+            const bool in = r < N && c < N;
+            const int ix = in ? r * N + c : 0;
+            const double2 t = A.rin[base + ix];
+            xr[q] = in ? t : make_double2(0.0, 0.0);
+            if (VLDS) {
+                const double2 u = V[ix];
+                xv[q] = in ? u : make_double2(0.0, 0.0);
             }
         }
+#pragma unroll
+        for (int t = 0; t < TPW; ++t) {
+            const int tt = wave * TPW + t, ti = tt / NB, tj = tt - (tt / NB) * NB;
+            const int col = tj * 16 + li;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = ti * 16 + lk + 4 * r;
+                const size_t gi = (size_t)(base + (row < N && col < N ? row * N + col : 0));
+                yv[t][r] = yp[gi];
+                av[t][r] = ap[gi];
+            }
+        }
+        gather_data<2, 2>(A, I, 1, threadIdx.x, 256, nbx, sC);
+    } else {
+#pragma unroll
+        for (int q = 0; q < SPT; ++q) {
+            const int e = threadIdx.x + q * 256;
+            const int r = e / NP, c = e - r * NP;
+            const bool in = r < N && c < N;
+            xr[q] = in ? A.rin[base + r * N + c] : make_double2(0.0, 0.0);
+            if (VLDS) xv[q] = in ? V[r * N + c] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int t = 0; t < TPW; ++t) {
+            const int tt = wave * TPW + t, ti = tt / NB, tj = tt - (tt / NB) * NB;
+            const int col = tj * 16 + li;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = ti * 16 + lk + 4 * r;
+                yv[t][r] = make_double2(0.0, 0.0);
+                av[t][r] = make_double2(0.0, 0.0);
+                if (row < N && col < N && A.stage != 0) {
+                    const size_t gi = (size_t)(base + row * N + col);
+                    if (A.stage < 3) yv[t][r] = A.y[gi];
+                    av[t][r] = A.acc[gi];
+                }
+            }
+        }
+        gather_couplings(A, I, a0, 1, sC, threadIdx.x, 256);
     }
-    gather_couplings(A, I, a0, 1, sC, threadIdx.x, 256);
 #pragma unroll
     for (int q = 0; q < SPT; ++q) {
         const int e = threadIdx.x + q * 256;
@@ -836,6 +872,7 @@ int launch_cfg(const Group& g, const LaunchCfg& c, const StageArgs& base, hipStr
         EH_CASE(9) EH_CASE(10) EH_CASE(11) EH_CASE(12) EH_CASE(13) EH_CASE(14) EH_CASE(15) EH_CASE(16)
 #undef EH_CASE
         case -32: hipLaunchKernelGGL((stage_mfma<32, true>), grid, block, c.shmem, s, a); break;
+        case -34: hipLaunchKernelGGL((stage_mfma<32, true, true>), grid, block, c.shmem, s, a); break;
         case -64: hipLaunchKernelGGL((stage_mfma<64, false>), grid, block, c.shmem, s, a); break;
         case -65: hipLaunchKernelGGL(stage_mfma64, grid, dim3(512), c.shmem, s, a); break;
         case -33:
@@ -1140,6 +1177,7 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
         g.persist = 2 * ctx->n_cu;
         int maxnn = 0, maxkn = 0, maxd = 0;
         bool pipe_ok = cls == 17;
+        bool tp_ok = true;   // two-phase gathers: 2 K N <= 512 and K_m <= 2 (and a bath) everywhere
         const InstDev& f0 = hinst[(size_t)members[0]];
         for (int i : members) {
             const InstDev& di = hinst[(size_t)i];
@@ -1148,6 +1186,7 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
             maxd = std::max(maxd, di.d);
             // the pipelined kernel takes the RK coefficients as arguments: one time step per group
             if (di.Km > 2 || 2 * di.K * di.N > 512 || di.h != f0.h) pipe_ok = false;
+            if (di.Km > 2 || di.Km < 1 || di.K < 1 || 2 * di.K * di.N > 512) tp_ok = false;
         }
         g.maxnn = maxnn;
         g.maxkn = maxkn;
@@ -1185,7 +1224,7 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
             c.tile_count = r.second;
             if (nt == -33)
                 c.shmem = (size_t)2 * 32 * 33 * sizeof(double2) + (size_t)2 * 32 * 33 * sizeof(double) + 2 * cpb;
-            else if (nt == -32)
+            else if (nt == -32 || nt == -34)
                 c.shmem = (size_t)2 * 32 * 33 * sizeof(double2) + cpb;
             else if (nt == -64)
                 c.shmem = (size_t)64 * 65 * sizeof(double2) + cpb;
@@ -1215,11 +1254,13 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
             add(cls, 256, false);
             add(cls, 1024, false);
             add(0, 0, false);
-        } else if (cls == 17) {
+        } else if (cls == 17) {   // base: two-phase gathers with column copies where the instances allow
+            if (use_mfma && tp_ok) add(-34, 0, true);
             if (use_mfma) add(-32, 0, true);
             add(0, 0, true);
             if (use_mfma) add(-32, 0, false);
             if (pipe_ok) add(-33, 0, false);
+            if (use_mfma && tp_ok) add(-34, 0, false);
         } else {   // 32 < N <= 64: stage_mfma64 gathers columns directly (measured faster at N = 56)
             if (use_mfma) add(-65, 0, false);
             add(0, 0, true);
@@ -1482,6 +1523,7 @@ std::string batch_tuning_report(const Batch* b) {
             const LaunchCfg& x = g.cand[c];
             const char* kn = x.nt > 0      ? "stage_small"
                              : x.nt == -32 ? "stage_mfma32"
+                             : x.nt == -34 ? "stage_mfma32_tp"
                              : x.nt == -64 ? "stage_mfma64_v1"
                              : x.nt == -65 ? "stage_mfma64"
                              : x.nt == -33 ? "stage_mfma_pipe"
