@@ -52,6 +52,8 @@ struct AdjArgs {
     const long long* gacc_off;
     int ncopy;               // tile t adds into copy t % ncopy (spreads the atomics)
     long long traj;          // trajectory mode: Yf moves by step * traj elements
+    double2* slots;          // small-N tiles: one accumulator block per tile (plain read-modify-write)
+    const long long* slot_off;
 };
 
 // layout of one instance's accumulator block (complex entries)
@@ -132,9 +134,39 @@ __device__ void reduce_bath(const AdjArgs& J, const InstDev& I, int n0, int na, 
 // of the tile's ADO (kb from LDS) against the forward stage input of the up and
 // down neighbours, then adds its two results into the LDS sums (x: U_p;  y: A_p
 // for rows, B_p for columns; p = j + k K).
+// Forward-table values of the thread's first (ADO, bath, row/column, mode) slot,
+// loaded at kernel entry (the reduction itself runs after the stage update).
+struct BathSlot {
+    int mu, md;
+    double x, y;
+};
+
+__device__ __forceinline__ int bath_slot_q(const InstDev& I, int n0, int t, int& p) {
+    const int K = I.K, Km = I.Km;
+    const int k = t % Km, r1 = t / Km;
+    const int r2 = r1 / 2;
+    const int j = r2 % K, g = r2 / K;
+    p = j + k * K;
+    return (n0 + g) * I.d + p;
+}
+
+__device__ __forceinline__ BathSlot bath_slot_load(const AdjArgs& J, const InstDev& I, int n0, int na, int tid) {
+    const int nt = na * I.K * 2 * I.Km;
+    int p;
+    const int t = tid < nt ? tid : 0;
+    const long long q = I.lk_off + bath_slot_q(I, n0, t, p);
+    BathSlot b;
+    b.mu = J.fnb_up[q];
+    b.md = J.fnb_dn[q];
+    b.x = J.fcx[q];
+    b.y = J.fcy[q];
+    return b;
+}
+
 template <int NT>
 __device__ __forceinline__ void reduce_bath_rows(const AdjArgs& J, const InstDev& I, int n0, int na,
-                                                 const double2* sKb, double fs, double* sU, int tid, int nthr) {
+                                                 const double2* sKb, double fs, double* sU, int tid, int nthr,
+                                                 const BathSlot& first) {
     constexpr int NN = NT * NT;
     const int K = I.K, Km = I.Km, d = I.d;
     const int nt = na * K * 2 * Km;
@@ -144,8 +176,9 @@ __device__ __forceinline__ void reduce_bath_rows(const AdjArgs& J, const InstDev
         const int j = r2 % K, g = r2 / K;
         const int s = j + I.so, p = j + k * K;
         const long long q = I.lk_off + (long long)(n0 + g) * d + p;
-        const int mu = J.fnb_up[q], md = J.fnb_dn[q];
-        const double x = J.fcx[q], y = J.fcy[q];
+        const bool pre = t == tid;   // the first slot's table values came with the tile's loads
+        const int mu = pre ? first.mu : J.fnb_up[q], md = pre ? first.md : J.fnb_dn[q];
+        const double x = pre ? first.x : J.fcx[q], y = pre ? first.y : J.fcy[q];
         const double2* kb = sKb + g * NN;
         const double2* Yu = J.Yf + I.state_off + (long long)(mu > 0 ? mu : 0) * NN;
         const double2* Yd = J.Yf + I.state_off + (long long)(md > 0 ? md : 0) * NN;
@@ -202,6 +235,7 @@ __global__ __launch_bounds__(256) void adj_small(AdjArgs J) {
     __shared__ double2 sG[G * NN];
     __shared__ double2 sQ[G * NN];
     __shared__ double sU[3 * 2 * DMAX];
+    __shared__ double sSg[2 * DMAX];
     const int tile = blockIdx.x;
     const int ii = A.tile_inst[tile];
     // a register copy: read once before any store (a reference would be re-read after them)
@@ -220,6 +254,12 @@ __global__ __launch_bounds__(256) void adj_small(AdjArgs J) {
     NbIdx<2, 2> nbx;
     if (has_cpl) gather_idx<2, 2, NT>(A, I, a0, na, threadIdx.x, 256, nbx);
     const bool vreal = A.vreal[I.step_off + step] != 0;
+    // tail loads issued with the tile's: forward-table slot of the bath reduction and
+    // the occupation n_p of the damping-gradient slot (t = n d + p)
+    BathSlot bslot;
+    if (has_cpl) bslot = bath_slot_load(J, I, a0, na, threadIdx.x);
+    const int dq = na * I.d;
+    const int np_pre = dq > 0 ? J.ado[I.lk_off + (long long)a0 * I.d + (threadIdx.x < dq ? threadIdx.x : 0)] : 0;
     const double2* lp = (st == 1 || st == 2) ? A.y : A.rin;
     const double2* ap = st > 0 ? A.acc : A.rin;
     double2 rv[EPT], lv[EPT], av[EPT], yf[EPT], gv[EPT];
@@ -245,6 +285,7 @@ __global__ __launch_bounds__(256) void adj_small(AdjArgs J) {
         if (total > 2 * 256 || I.Km > 2) gather_rest<2, 2>(A, I, a0, na, threadIdx.x, 256, sC);
     }
     for (int e = threadIdx.x; e < 6 * I.d; e += 256) sU[e] = 0.0;
+    for (int e = threadIdx.x; e < 2 * I.d; e += 256) sSg[e] = 0.0;
 #pragma unroll
     for (int q = 0; q < VPT; ++q) {
         const int e = threadIdx.x + q * 256;
@@ -309,40 +350,67 @@ __global__ __launch_bounds__(256) void adj_small(AdjArgs J) {
         rk_store_adj(A, I, (size_t)(base + e), lam, st == 0 ? make_double2(0.0, 0.0) : av[q], ur + cpl.x,
                      ui + cpl.y);
     }
-    reduce_bath_rows<NT>(J, I, a0, na, sR, fs, sU, threadIdx.x, 256);
+    if (has_cpl) reduce_bath_rows<NT>(J, I, a0, na, sR, fs, sU, threadIdx.x, 256, bslot);
     __syncthreads();
-    double2* acc = J.gacc + J.gacc_off[ii] + (long long)(tile % J.ncopy) * gacc_size(NN, I.d);
+    // this tile's own accumulator block: no other workgroup touches it in this launch,
+    // and launches are ordered, so a plain (cached) read-modify-write suffices
+    double2* acc = J.slots + J.slot_off[tile];
     const double2* ef = J.field + 3 * (size_t)(I.step_off + step);
     const double2 e0 = ef[0], e1 = ef[1], e2 = ef[2];
     const bool field_on = e0.x != 0.0 || e0.y != 0.0 || e1.x != 0.0 || e1.y != 0.0 || e2.x != 0.0 || e2.y != 0.0;
+    auto add2 = [](double2* p, double2 v) {
+        const double2 o = *p;
+        *p = make_double2(o.x + v.x, o.y + v.y);
+    };
     for (int ab = threadIdx.x; ab < NN; ab += 256) {
         double2 gs = make_double2(0.0, 0.0);
         for (int n = 0; n < na; ++n) {
             gs.x += sG[n * NN + ab].x;
             gs.y += sG[n * NN + ab].y;
         }
-        atomic_add2(acc + ab, gs);
+        add2(acc + ab, gs);
         if (field_on) {   // G_q = sum e_q Gt: nothing to add outside the pulse windows
-            atomic_add2(acc + NN + ab, cmad(make_double2(0.0, 0.0), e0, gs));
-            atomic_add2(acc + 2 * NN + ab, cmad(make_double2(0.0, 0.0), e1, gs));
-            atomic_add2(acc + 3 * NN + ab, cmad(make_double2(0.0, 0.0), e2, gs));
+            add2(acc + NN + ab, cmad(make_double2(0.0, 0.0), e0, gs));
+            add2(acc + 2 * NN + ab, cmad(make_double2(0.0, 0.0), e1, gs));
+            add2(acc + 3 * NN + ab, cmad(make_double2(0.0, 0.0), e2, gs));
         }
     }
     const int d = I.d;
-    for (int t = threadIdx.x; t < na * d; t += 256) {
+    // q_n = sum_ab sQ[n][ab] in two levels (8 partial sums per ADO), then n_p q_n per (n, p)
+    constexpr int QP = NN < 8 ? NN : 8, QC = (NN + QP - 1) / QP;   // na * QP <= G * NN: fits sG
+    double2* sPart = sG;   // the Gt partials are consumed above; reuse their LDS
+    __syncthreads();
+    for (int t = threadIdx.x; t < na * QP; t += 256) {
+        const int n = t / QP, part = t - n * QP;
+        double2 qs = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int c = 0; c < QC; ++c) {
+            const int ab = part * QC + c;
+            if (ab < NN) {
+                qs.x += sQ[n * NN + ab].x;
+                qs.y += sQ[n * NN + ab].y;
+            }
+        }
+        sPart[t] = qs;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < na * d; t += 256) {   // n_p q_n summed over the tile's ADOs in LDS
         const int n = t / d, p = t - n * d;
-        const int np_ = J.ado[I.lk_off + (long long)(a0 + n) * d + p];
+        const int np_ = t == (int)threadIdx.x ? np_pre : J.ado[I.lk_off + (long long)(a0 + n) * d + p];
         if (np_ == 0) continue;
         double2 qn = make_double2(0.0, 0.0);
-        for (int ab = 0; ab < NN; ++ab) {
-            qn.x += sQ[n * NN + ab].x;
-            qn.y += sQ[n * NN + ab].y;
+#pragma unroll
+        for (int part = 0; part < QP; ++part) {
+            qn.x += sPart[n * QP + part].x;
+            qn.y += sPart[n * QP + part].y;
         }
-        atomic_add2(acc + 5 * NN + p, make_double2(np_ * qn.x, np_ * qn.y));
+        atomicAdd(&sSg[2 * p], np_ * qn.x);
+        atomicAdd(&sSg[2 * p + 1], np_ * qn.y);
     }
-    for (int t = threadIdx.x; t < 3 * d; t += 256) {
-        const int w = t / d, p = t - w * d;
-        atomic_add2(acc + 5 * NN + (1 + w) * d + p, make_double2(sU[2 * (w * d + p)], sU[2 * (w * d + p) + 1]));
+    __syncthreads();
+    for (int t = threadIdx.x; t < 4 * d; t += 256) {   // S_gamma, U, A, B
+        const double* src = t < d ? sSg + 2 * t : sU + 2 * (t - d);
+        add2(acc + 5 * NN + t, make_double2(src[0], src[1]));
     }
 }
 
@@ -540,7 +608,11 @@ __global__ void set_ctr(int* ctr, int v) {
 
 struct AdjState {
     int C = 0;                 // checkpoint interval (steps per segment)
-    int ncopy = 32;            // accumulator copies per instance
+    int ncopy = 32;            // accumulator copies per instance (N > 16 kernels: atomics)
+    DevBuf<double2> slots;     // small-N tiles: one accumulator block each
+    DevBuf<long long> slot_off;
+    std::vector<long long> h_slot_off;   // per tile (-1: not a small-N tile)
+    std::vector<double2> h_slots;
     int nseg = 0;
     long long total = 0;       // state elements of the batch
     bool trajm = false;        // trajectory mode: every stage input of the run is kept, no replay
@@ -574,6 +646,7 @@ static int launch_adj(const Group& g, const AdjArgs& base, hipStream_t s) {
     a.S.tile_inst += g.tile_begin;
     a.S.tile_ado0 += g.tile_begin;
     a.S.tile_nado += g.tile_begin;
+    a.slot_off += g.tile_begin;
     const dim3 grid(g.tile_count), block(256);
     switch (g.nt) {
 #define EH_CASE(n) \
@@ -634,6 +707,17 @@ int batch_enable_adjoint(Batch* b, int C) {
     }
     for (size_t e = 0; e < adamp.size(); e++) adamp[e] = make_double2(im->hdamp[e].x, -im->hdamp[e].y);
     a->h_gacc_off = goff;
+    // one accumulator block per tile of the small-N groups (their base tiling)
+    a->h_slot_off.assign(im->htile_inst.size(), -1);
+    long long stot = 0;
+    for (const Group& g : im->groups) {
+        if (g.nt < 1 || g.nt > 16) continue;
+        for (int t = g.tile_begin; t < g.tile_begin + g.tile_count; t++) {
+            const InstDev& I = im->hinst[(size_t)im->htile_inst[(size_t)t]];
+            a->h_slot_off[(size_t)t] = stot;
+            stot += gacc_size(I.N * I.N, I.d);
+        }
+    }
     int rc;
     const size_t T = (size_t)a->total;
     // Trajectory mode when every stage input of the run fits in half of the free
@@ -654,7 +738,8 @@ int batch_enable_adjoint(Batch* b, int C) {
                   : ((rc = a->ckpt.alloc(T * (size_t)(a->nseg + 1))) || (rc = a->stg.alloc(T * 4 * (size_t)(a->C + 1))) ||
                      (rc = a->acc_f.alloc(T)))) ||
         (rc = a->lam.alloc(T)) || (rc = a->aacc.alloc(T)) || (rc = a->q1.alloc(T)) || (rc = a->q2.alloc(T)) ||
-        (rc = a->VH.alloc(im->V.n)) || (rc = a->gP.alloc(im->P.n)) || (rc = a->gacc.alloc((size_t)gtot)))
+        (rc = a->VH.alloc(im->V.n)) || (rc = a->gP.alloc(im->P.n)) || (rc = a->gacc.alloc((size_t)gtot)) ||
+        (rc = a->slots.alloc((size_t)stot)) || (rc = a->slot_off.upload(a->h_slot_off)))
         return rc;
     const int blocks = (int)std::min<long long>(std::max<long long>(1, ((long long)T + 255) / 256), 65536);
     hipStream_t s = im->stream;
@@ -718,6 +803,8 @@ int batch_enable_adjoint(Batch* b, int C) {
     aj.gacc = a->gacc.p;
     aj.gacc_off = a->gacc_off.p;
     aj.ncopy = a->ncopy;
+    aj.slots = a->slots.p;
+    aj.slot_off = a->slot_off.p;
     AdjPrologueArgs ap;
     ap.inst = im->inst.p;
     ap.step_ctr = im->step_ctr.p;
@@ -817,6 +904,7 @@ int batch_backward(Batch* b, const double* dP) {
     const int blocks = (int)std::min<long long>(std::max<long long>(1, (T + 255) / 256), 65536);
     hipLaunchKernelGGL(fill_zero, dim3(blocks), dim3(256), 0, s, a->lam.p, T);
     hipLaunchKernelGGL(fill_zero, dim3(256), dim3(256), 0, s, a->gacc.p, (long long)a->gacc.n);
+    if (a->slots.n) hipLaunchKernelGGL(fill_zero, dim3(256), dim3(256), 0, s, a->slots.p, (long long)a->slots.n);
     hipLaunchKernelGGL(set_ctr, dim3(1), dim3(1), 0, s, im->step_ctr.p, (a->nseg - 1) * a->C);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(im->ev0, s));
@@ -834,6 +922,22 @@ int batch_backward(Batch* b, const double* dP) {
                 blk[e].x += blk[c * gs + e].x;
                 blk[e].y += blk[c * gs + e].y;
             }
+    }
+    if (a->slots.n) {   // and the per-tile blocks of the small-N tiles into their instance's
+        a->h_slots.resize(a->slots.n);
+        HIPCHK(hipMemcpy(a->h_slots.data(), a->slots.p, a->slots.n * sizeof(double2), hipMemcpyDeviceToHost));
+        for (size_t t = 0; t < a->h_slot_off.size(); t++) {
+            if (a->h_slot_off[t] < 0) continue;
+            const int i = im->htile_inst[t];
+            const InstDev& I = im->hinst[(size_t)i];
+            const long long gs = gacc_size(I.N * I.N, I.d);
+            double2* blk = a->h_gacc.data() + a->h_gacc_off[(size_t)i];
+            const double2* src = a->h_slots.data() + a->h_slot_off[t];
+            for (long long e = 0; e < gs; e++) {
+                blk[e].x += src[e].x;
+                blk[e].y += src[e].y;
+            }
+        }
     }
     a->have_grad = true;
     return EH_OK;

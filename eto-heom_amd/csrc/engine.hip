@@ -1208,7 +1208,9 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
                 }
             }
             const std::pair<int, int> r(begin, (int)t_inst.size() - begin);
-            if (xcd > 1) xcd_order(begin, r.second);
+            // measured: helps the N > 16 kernels; the small-N forward and the adjoint
+            // (atomics of one instance from one XCD) are faster without it
+            if (xcd > 1 && cls > 16) xcd_order(begin, r.second);
             ranges.push_back({G, r});
             return r;
         };
@@ -1305,6 +1307,7 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
     im->halt = std::move(halt);
     im->hdamp = std::move(hdamp);
     im->hado = std::move(hado);
+    im->htile_inst = t_inst;
     HIPCHK(hipStreamCreateWithFlags(&im->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&im->ev0));
     HIPCHK(hipEventCreate(&im->ev1));

@@ -103,6 +103,7 @@ struct BatchImpl {
     std::vector<double> hcx, hcy;
     std::vector<double2> hal, halt, hdamp;
     std::vector<uint8_t> hado;           // multi-indices, same (ADO, p) layout as the tables
+    std::vector<int> htile_inst;         // instance of every tile
     AdjState* adj = nullptr;
 
     ~BatchImpl() {

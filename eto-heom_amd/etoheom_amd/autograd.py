@@ -40,6 +40,7 @@ class Spec:
     nv: list = field(default_factory=list)   # (B,) tuples
     checkpoint: int = 32
     site_offset: int = 1
+    timing: dict = field(default_factory=dict)   # filled: GPU-event ms of the last forward / backward
 
 
 class _Propagate(torch.autograd.Function):
@@ -68,7 +69,8 @@ class _Propagate(torch.autograd.Function):
             P[i, : len(o)] = o
         ctx.batch = b
         ctx.lens = lens
-        ctx.elapsed_ms = b.elapsed_ms()
+        ctx.spec = spec
+        spec.timing["forward_ms"] = b.elapsed_ms()
         return torch.from_numpy(P)
 
     @staticmethod
@@ -76,7 +78,9 @@ class _Propagate(torch.autograd.Function):
         b = ctx.batch
         g = gP.detach().cpu().numpy().astype(np.complex128)
         b.backward([g[i, :n] for i, n in enumerate(ctx.lens)])
+        ctx.spec.timing["backward_ms"] = b.elapsed_ms()
         grads = [b.grad(i) for i in range(len(ctx.lens))]
+        ctx.batch = None   # release the device state (the trajectory) with the graph
 
         def stack(name):
             return torch.from_numpy(np.stack([gr[name] for gr in grads]))
