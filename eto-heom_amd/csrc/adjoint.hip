@@ -683,9 +683,12 @@ int batch_enable_adjoint(Batch* b, int C) {
     std::vector<double2> aal(im->hal.size()), aalt(im->halt.size()), adamp(im->hdamp.size());
     std::vector<long long> goff;
     long long gtot = 0;
+    std::vector<long long> done;   // instances share hierarchy tables: build each once
     for (int i = 0; i < b->n_inst; i++) {
         const InstDev& I = im->hinst[(size_t)i];
-        for (int m = 0; m < I.M; m++)
+        const bool seen = std::find(done.begin(), done.end(), I.lk_off) != done.end();
+        if (!seen) done.push_back(I.lk_off);
+        for (int m = 0; m < (seen ? 0 : I.M); m++)
             for (int p = 0; p < I.d; p++) {
                 const size_t q = (size_t)(I.lk_off + (long long)m * I.d + p);
                 const int j = p % I.K, k = p / I.K;

@@ -31,6 +31,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <type_traits>
@@ -1007,6 +1008,10 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
     std::vector<int> hup, hdn, hpidx;
     std::vector<uint8_t> hado;
     std::vector<double> hcx, hcy;
+    // Hierarchy tables (neighbour indices, occupations, link coefficients) once per
+    // distinct (d, L, alpha): they depend on nothing else, so a grid of propagations
+    // with one bath shares a few small, cache-resident tables.
+    std::map<std::vector<double>, int64_t> hier_off;
     std::vector<uint8_t> hvreal;
     std::vector<double2> hH, hXp, hfield, hR, hal, halt, hdamp;
     std::vector<long long> hado0;
@@ -1074,11 +1079,24 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
             hvreal.push_back((real_h && !s.active[(size_t)st]) ? 1 : 0);
         }
         for (int m = 0; m < M; m++) hdamp.push_back(d2(tb.damp[(size_t)m]));
-        hup.insert(hup.end(), tb.up.begin(), tb.up.end());
-        hado.insert(hado.end(), h.ado.begin(), h.ado.end());
-        hdn.insert(hdn.end(), tb.dn.begin(), tb.dn.end());
-        hcx.insert(hcx.end(), tb.cx.begin(), tb.cx.end());
-        hcy.insert(hcy.end(), tb.cy.begin(), tb.cy.end());
+        int64_t inst_lk;
+        {
+            std::vector<double> key = {(double)h.d, (double)sys.L, (double)sys.K, (double)sys.Km};
+            key.insert(key.end(), sys.alpha, sys.alpha + 2 * sys.K * sys.Km);
+            const auto it = hier_off.find(key);
+            if (it != hier_off.end()) {
+                inst_lk = it->second;
+            } else {
+                inst_lk = lk_off;
+                hier_off[key] = lk_off;
+                hup.insert(hup.end(), tb.up.begin(), tb.up.end());
+                hado.insert(hado.end(), h.ado.begin(), h.ado.end());
+                hdn.insert(hdn.end(), tb.dn.begin(), tb.dn.end());
+                hcx.insert(hcx.end(), tb.cx.begin(), tb.cx.end());
+                hcy.insert(hcy.end(), tb.cy.begin(), tb.cy.end());
+                lk_off += (int64_t)M * h.d;
+            }
+        }
         for (int e = 0; e < sys.K * sys.Km; e++) {
             hal.push_back(make_double2(sys.alpha[2 * e], sys.alpha[2 * e + 1]));
             halt.push_back(make_double2(sys.alpha_t[2 * e], sys.alpha_t[2 * e + 1]));
@@ -1095,7 +1113,7 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
         d.nread = nread;
         d.npoints = s.npoints;
         d.r_off = r_off;
-        d.lk_off = lk_off;
+        d.lk_off = inst_lk;
         d.d = h.d;
         d.K = sys.K;
         d.Km = sys.Km;
@@ -1130,7 +1148,6 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
         v_off += NN;
         r_off += nread * NN;
         step_off += s.nsteps;
-        lk_off += (int64_t)M * h.d;
         al_off += sys.K * sys.Km;
     }
     b->total_ados = ado_off;
