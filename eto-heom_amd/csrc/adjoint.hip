@@ -741,7 +741,7 @@ int batch_enable_adjoint(Batch* b, int C) {
                   : ((rc = a->ckpt.alloc(T * (size_t)(a->nseg + 1))) || (rc = a->stg.alloc(T * 4 * (size_t)(a->C + 1))) ||
                      (rc = a->acc_f.alloc(T)))) ||
         (rc = a->lam.alloc(T)) || (rc = a->aacc.alloc(T)) || (rc = a->q1.alloc(T)) || (rc = a->q2.alloc(T)) ||
-        (rc = a->VH.alloc(im->V.n)) || (rc = a->gP.alloc(im->P.n)) || (rc = a->gacc.alloc((size_t)gtot)) ||
+        (rc = a->VH.alloc(im->V.n / 2)) || (rc = a->gP.alloc(im->P.n)) || (rc = a->gacc.alloc((size_t)gtot)) ||
         (rc = a->slots.alloc((size_t)stot)) || (rc = a->slot_off.upload(a->h_slot_off)))
         return rc;
     const int blocks = (int)std::min<long long>(std::max<long long>(1, ((long long)T + 255) / 256), 65536);

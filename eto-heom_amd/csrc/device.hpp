@@ -61,6 +61,25 @@ struct StageArgs {
     // Trajectory mode: rin, out and y are slots of a per-step array and move by
     // step * traj elements (0: fixed buffers)
     long long traj;
+    // Fused steps (the forward graph): V is double-buffered by step parity (the odd
+    // steps' copy vstride / vpstride elements on), stage 0's workgroup holding ADO 0
+    // reads the instance out, and stage 3's builds V of the next step into the other
+    // copy, so no per-step prologue launch is needed.  0: one V, built by the prologue.
+    int fused;
+    long long vstride, vpstride;
+    const struct FusedDev* fz;   // what the fused tail reads (device memory: few kernel-argument registers)
+};
+
+struct FusedDev {
+    double2* V;              // copy 0 of V and of Vpad (the odd steps' copies vstride / vpstride on)
+    double* Vpad;
+    long long vstride, vpstride;
+    const double2* H;        // per instance N*N
+    const double2* Xp;       // per instance 3*N*N: X[0], X[1]^T, X[2]^T
+    const double2* field;    // per step 3 scalars
+    const int* pidx;         // per step readout slot
+    const double2* R;        // per instance nread*N*N readout operators
+    double2* P;
 };
 
 // the stage's buffers at this step (trajectory mode)
@@ -70,6 +89,10 @@ __device__ __forceinline__ void at_step(StageArgs& A, int step) {
         A.rin += o;
         A.out += o;
         A.y += o;
+    }
+    if (A.fused && (step & 1)) {
+        A.V += A.vstride;
+        A.Vpad += A.vpstride;
     }
 }
 
@@ -343,73 +366,84 @@ struct PrologueArgs {
     long long traj;          // trajectory mode: y moves by step * traj elements
 };
 
-// V(t_i) = H - e_1 X[0] - e_2 X[1]^T - e_3 X[2]^T  (dynamics.cpp:348-363), and
-// the readouts Tr(rho_0 R_r) before step i (dynamics.cpp:367-381, :246-253)
-static __global__ __launch_bounds__(256) void prologue(PrologueArgs A) {
-    const InstDev& I = A.inst[A.ilist ? A.ilist[blockIdx.x] : (int)blockIdx.x];
-    const int step = *A.step_ctr + A.step_off;
-    if (step >= I.nsteps) return;
+// V(t_i) = H - e_1 X[0] - e_2 X[1]^T - e_3 X[2]^T  (dynamics.cpp:348-363) with e the
+// three field scalars of step i, by the whole block; the zero-padded 64x64 real and
+// imaginary planes as well when Vpad is not null (instances with N > 32).
+__device__ __forceinline__ void build_v(const InstDev& I, const double2* H, const double2* X, const double2* e,
+                                        double2* V, double* Vpad) {
     const int N = I.N, NN = N * N;
-    const double2* e = A.field + 3 * (size_t)(I.step_off + step);
     const double2 e0 = e[0], e1 = e[1], e2 = e[2];
-    const double2* H = A.H + I.v_off;
-    const double2* X = A.Xp + 3 * (size_t)I.v_off;
-    double2* V = A.V + I.v_off;
-    for (int q = threadIdx.x; q < NN; q += blockDim.x) {
-        double2 v = H[q];
-        const double2 x0 = X[q], x1 = X[NN + q], x2 = X[2 * NN + q];
+    auto vat = [&](int ix) {
+        double2 v = H[ix];
+        const double2 x0 = X[ix], x1 = X[NN + ix], x2 = X[2 * NN + ix];
         v.x -= x0.x * e0.x - x0.y * e0.y;
         v.y -= x0.x * e0.y + x0.y * e0.x;
         v.x -= x1.x * e1.x - x1.y * e1.y;
         v.y -= x1.x * e1.y + x1.y * e1.x;
         v.x -= x2.x * e2.x - x2.y * e2.y;
         v.y -= x2.x * e2.y + x2.y * e2.x;
-        V[q] = v;
-    }
-    if (I.vp_off >= 0) {   // the same V, zero-padded to 64 x 64 as real and imaginary planes
-        double* vr = A.Vpad + I.vp_off;
-        double* vi = vr + 64 * 64;
+        return v;
+    };
+    for (int q = threadIdx.x; q < NN; q += blockDim.x) V[q] = vat(q);
+    if (Vpad) {
+        double* vi = Vpad + 64 * 64;
         for (int q = threadIdx.x; q < 64 * 64; q += blockDim.x) {
             const int r = q >> 6, c = q & 63;
-            double2 v = make_double2(0.0, 0.0);
-            if (r < N && c < N) {
-                const int ix = r * N + c;
-                v = H[ix];
-                const double2 x0 = X[ix], x1 = X[NN + ix], x2 = X[2 * NN + ix];
-                v.x -= x0.x * e0.x - x0.y * e0.y;
-                v.y -= x0.x * e0.y + x0.y * e0.x;
-                v.x -= x1.x * e1.x - x1.y * e1.y;
-                v.y -= x1.x * e1.y + x1.y * e1.x;
-                v.x -= x2.x * e2.x - x2.y * e2.y;
-                v.y -= x2.x * e2.y + x2.y * e2.x;
-            }
-            vr[q] = v.x;
+            const double2 v = (r < N && c < N) ? vat(r * N + c) : make_double2(0.0, 0.0);
+            Vpad[q] = v.x;
             vi[q] = v.y;
         }
     }
-    const int slot = A.pidx[I.step_off + step];
-    if (slot < 0 || !A.readout) return;
-    __shared__ double2 red[256];
-    const double2* rho0 = A.y + (long long)step * A.traj + I.state_off;
+}
+
+// The readouts Tr(rho_0 R_r) before a step (dynamics.cpp:367-381, :246-253) into
+// P[r * npoints + slot], by lanes 0..63 of the block (wave 0, every lane present).
+// One summation order wherever it runs, so every path gives the same bits.
+__device__ __forceinline__ void readout_wave(const InstDev& I, const double2* rho0, const double2* R, int slot,
+                                             double2* P) {
+    const int N = I.N, NN = N * N, lane = threadIdx.x;
     for (int r = 0; r < I.nread; ++r) {
-        const double2* R = A.R + I.r_off + (size_t)r * NN;
+        const double2* Rr = R + (size_t)r * NN;
         double2 s = make_double2(0.0, 0.0);
-        for (int q = threadIdx.x; q < NN; q += blockDim.x) {
+        for (int q = lane; q < NN; q += 64) {
             const int j = q / N, k = q - j * N;
-            s = cmad(s, rho0[q], R[k * N + j]);
+            s = cmad(s, rho0[q], Rr[k * N + j]);
         }
-        red[threadIdx.x] = s;
-        __syncthreads();
-        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) {
-                red[threadIdx.x].x += red[threadIdx.x + w].x;
-                red[threadIdx.x].y += red[threadIdx.x + w].y;
-            }
-            __syncthreads();
+        for (int o = 32; o > 0; o >>= 1) {
+            s.x += __shfl_xor(s.x, o, 64);
+            s.y += __shfl_xor(s.y, o, 64);
         }
-        if (threadIdx.x == 0) A.P[I.p_off + (long long)r * I.npoints + slot] = red[0];
-        __syncthreads();
+        if (lane == 0) P[(long long)r * I.npoints + slot] = s;
     }
+}
+
+// End of a fused stage, in the workgroup whose tile starts at the instance's ADO 0:
+// stage 0 reads out rho_0 before the step (its input, which no stage-0 workgroup
+// writes); stage 3 builds V of the next step into the copy the step after reads
+// (that copy was last read by the previous step's stages, all finished).
+__device__ __forceinline__ void fused_tail(const StageArgs& A, const InstDev& I, int step) {
+    const FusedDev& F = *A.fz;
+    if (A.stage == 0) {
+        const int slot = F.pidx[I.step_off + step];
+        if (slot >= 0 && threadIdx.x < 64) readout_wave(I, A.rin + I.state_off, F.R + I.r_off, slot, F.P + I.p_off);
+    } else if (A.stage == 3 && step + 1 < I.nsteps) {
+        const long long o = ((step + 1) & 1) ? 1 : 0;
+        build_v(I, F.H + I.v_off, F.Xp + 3 * (size_t)I.v_off, F.field + 3 * (size_t)(I.step_off + step + 1),
+                F.V + o * F.vstride + I.v_off, I.vp_off >= 0 ? F.Vpad + o * F.vpstride + I.vp_off : nullptr);
+    }
+}
+
+// V(t_i) (build_v) and the readouts before step i (readout_wave) as a kernel of its
+// own, one workgroup per instance: the per-launch and adjoint paths.
+static __global__ __launch_bounds__(256) void prologue(PrologueArgs A) {
+    const InstDev& I = A.inst[A.ilist ? A.ilist[blockIdx.x] : (int)blockIdx.x];
+    const int step = *A.step_ctr + A.step_off;
+    if (step >= I.nsteps) return;
+    build_v(I, A.H + I.v_off, A.Xp + 3 * (size_t)I.v_off, A.field + 3 * (size_t)(I.step_off + step), A.V + I.v_off,
+            I.vp_off >= 0 ? A.Vpad + I.vp_off : nullptr);
+    const int slot = A.pidx[I.step_off + step];
+    if (slot < 0 || !A.readout || threadIdx.x >= 64) return;
+    readout_wave(I, A.y + (long long)step * A.traj + I.state_off, A.R + I.r_off, slot, A.P + I.p_off);
 }
 
 static __global__ void tick(int* ctr, int by) {

@@ -146,6 +146,7 @@ __global__ __launch_bounds__(256) void stage_small(StageArgs A) {
         rk_store(A, I, (size_t)(base + e), st == 0 ? r : yv[q], st == 0 ? make_double2(0.0, 0.0) : av[q], dr, di,
                  a0 + nl, ra, cb);
     }
+    if (A.fused && a0 == 0) fused_tail(A, I, step);
 }
 
 // any N <= 64: one ADO per workgroup, rho_n staged in dynamic LDS, V read through the cache
@@ -197,6 +198,7 @@ __global__ __launch_bounds__(256) void stage_generic(StageArgs A) {
         rk_load(A, (size_t)(base + ab), r, yv, av);
         rk_store(A, I, (size_t)(base + ab), yv, av, dr + cpl.x, di + cpl.y, a0, ra, cb);
     }
+    if (A.fused && a0 == 0) fused_tail(A, I, step);
 }
 
 // N in (16, NP]: one ADO per workgroup, the commutator on FP64 MFMA 16x16x4 tiles.
@@ -367,6 +369,7 @@ __global__ __launch_bounds__(256, NP == 32 ? (TP ? 3 : 4) : 2) void stage_mfma(S
             }
         }
     }
+    if (A.fused && a0 == 0) fused_tail(A, I, step);
 }
 
 // rho_n[r][c] from its N x (N+1) LDS copy, 0 in the padding up to 64 (clamped read, selected)
@@ -500,6 +503,7 @@ __global__ __launch_bounds__(512, 2) void stage_mfma64(StageArgs A) {
             }
         }
     }
+    if (A.fused && a0 == 0) fused_tail(A, I, step);
 }
 
 // rho_0 = |0><0| for every instance, all other ADOs zero; step counter to 0
@@ -919,6 +923,9 @@ StageArgs stage_args(const BatchImpl* im) {
     sa.gsign = 1.0;
     sa.y = im->Y.p;
     sa.acc = im->ACC.p;
+    sa.vstride = (long long)(im->V.n / 2);
+    sa.vpstride = (long long)(im->Vpad.n / 2);
+    sa.fz = im->fz.p;
     return sa;
 }
 
@@ -949,19 +956,27 @@ static void launch_reset(Batch* b) {
                        im->Yc.p, (long long)im->Yc.n, im->step_ctr.p);
     hipLaunchKernelGGL(set_ground, dim3((b->n_inst + 255) / 256), dim3(256), 0, im->stream, im->Y.p, im->Yc.p,
                        im->inst.p, b->n_inst);
+    PrologueArgs pa = prologue_args(im);   // V of step 0 (the fused graph's first step reads it)
+    pa.readout = 0;
+    hipLaunchKernelGGL(prologue, dim3(b->n_inst), dim3(256), 0, im->stream, pa);
 }
 
 // The forward graph: `chunk` steps, each group on its own branch (instances of
-// different groups are independent, so their prologue + stage chains overlap),
-// joined before the step-counter tick.
+// different groups are independent, so their stage chains overlap), joined before
+// the step-counter tick.  A step is four stage launches per group: the readouts
+// and the next step's V are done inside stages 0 and 3 (fused steps), except with
+// the persistent pipelined kernel, whose steps keep the prologue launch.
 static int capture_forward(Batch* b) {
     BatchImpl* im = b->impl;
     if (im->exec) (void)hipGraphExecDestroy(im->exec);
     if (im->graph) (void)hipGraphDestroy(im->graph);
     im->exec = nullptr;
     im->graph = nullptr;
-    const StageArgs sa = stage_args(im);
+    StageArgs sa = stage_args(im);
     const PrologueArgs pa = prologue_args(im);
+    bool fused = true;
+    for (const Group& g : im->groups) fused = fused && g.cand[g.sel].nt != -33;
+    sa.fused = fused ? 1 : 0;
     HIPCHK(hipStreamBeginCapture(im->stream, hipStreamCaptureModeThreadLocal));
     int rc = EH_OK;
     hipError_t e = hipEventRecord(im->fork_ev, im->stream);
@@ -973,7 +988,7 @@ static int capture_forward(Batch* b) {
         pg.ilist = im->ginst.p + g.inst_begin;
         for (int k = 0; k < b->chunk && rc == EH_OK && e == hipSuccess; k++) {
             pg.step_off = k;
-            hipLaunchKernelGGL(prologue, dim3(g.inst_count), dim3(256), 0, s, pg);
+            if (!fused) hipLaunchKernelGGL(prologue, dim3(g.inst_count), dim3(256), 0, s, pg);
             for (int st = 0; st < 4 && rc == EH_OK; st++) rc = launch_stage(g, stage_io(sa, im, k, st), s);
         }
         if (gi && e == hipSuccess) {
@@ -1309,12 +1324,24 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
         (rc = im->alpha_t.upload(halt)) || (rc = im->damp.upload(hdamp)) || (rc = im->ado0_off.upload(hado0)) ||
         (rc = im->ginst.upload(hginst)))
         return rc;
-    if ((rc = im->step_ctr.alloc(1)) || (rc = im->V.alloc(hH.size())) || (rc = im->P.alloc((size_t)p_off)) ||
+    if ((rc = im->step_ctr.alloc(1)) || (rc = im->V.alloc(2 * hH.size())) || (rc = im->P.alloc((size_t)p_off)) ||
         (rc = im->Y.alloc((size_t)state_off)) || (rc = im->ACC.alloc((size_t)state_off)) ||
         (rc = im->S1.alloc((size_t)state_off)) || (rc = im->S2.alloc((size_t)state_off)) ||
         (rc = im->Yc.alloc((size_t)col_off)) || (rc = im->S1c.alloc((size_t)col_off)) ||
-        (rc = im->S2c.alloc((size_t)col_off)) || (rc = im->Vpad.alloc((size_t)vp_off)))
+        (rc = im->S2c.alloc((size_t)col_off)) || (rc = im->Vpad.alloc(2 * (size_t)vp_off)))
         return rc;
+    FusedDev fz;
+    fz.V = im->V.p;
+    fz.Vpad = im->Vpad.p;
+    fz.vstride = (long long)(im->V.n / 2);
+    fz.vpstride = (long long)(im->Vpad.n / 2);
+    fz.H = im->H.p;
+    fz.Xp = im->Xp.p;
+    fz.field = im->field.p;
+    fz.pidx = im->pidx.p;
+    fz.R = im->R.p;
+    fz.P = im->P.p;
+    if ((rc = im->fz.upload(std::vector<FusedDev>{fz}))) return rc;
     im->hinst = hinst;
     im->hup = std::move(hup);
     im->hdn = std::move(hdn);
@@ -1339,7 +1366,7 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
         im->join_ev.push_back(e);
     }
     // graph of `chunk` steps
-    const int per_step = 5 * (int)im->groups.size();
+    const int per_step = 4 * (int)im->groups.size();
     b->chunk = std::max(1, std::min(b->max_steps, 256 / per_step));
     im->nchunks = (b->max_steps + b->chunk - 1) / b->chunk;
     b->impl = im.get();

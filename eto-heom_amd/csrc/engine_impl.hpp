@@ -87,6 +87,7 @@ struct BatchImpl {
     DevBuf<uint8_t> vreal;
     DevBuf<double> cx, cy;
     DevBuf<double2> V, H, Xp, field, R, P, alpha, alpha_t, damp, Y, ACC, S1, S2;
+    DevBuf<FusedDev> fz;          // the fused steps' pointers (V holds two copies, by step parity)
     DevBuf<double2> Yc, S1c, S2c;        // coupled-column copies of Y, S1, S2
     DevBuf<double> Vpad;                 // padded V planes of the N > 32 instances
     DevBuf<long long> ado0_off;
