@@ -199,12 +199,19 @@ def main():
     import torch
     import etoheom_amd as E
 
+    copy_gbs = None
     if a.copy_calibrate:
+        # a plain streaming copy of the same precision: the HBM rate this device reaches
         x = torch.ones(1 << 28, dtype=torch.float64, device="cuda")
         y = torch.empty_like(x)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.mul(x, 1.0, out=y)
+        e0.record()
         for _ in range(3):
             torch.mul(x, 1.0, out=y)
+        e1.record()
         torch.cuda.synchronize()
+        copy_gbs = 3 * 2 * x.numel() * 8 / (e0.elapsed_time(e1) * 1e-3) / 1e9
         del x, y
     if a.workload == "trimer_grad":
         return bench_gradient(a)
@@ -273,6 +280,8 @@ def main():
                      "kernel": "stage (fused RHS + RK4 stage)", "stage_ms": [float(x) for x in prof[1:5]],
                      "prologue_ms": float(prof[0]), "bytes_per_launch": bytes_per_launch},
     }
+    if copy_gbs:
+        rec["roofline"]["copy_gbs"] = copy_gbs
     if tuning:
         rec["config"]["autotune_steps"] = a.autotune
     if per_prop:

@@ -367,10 +367,10 @@ struct PrologueArgs {
 };
 
 // V(t_i) = H - e_1 X[0] - e_2 X[1]^T - e_3 X[2]^T  (dynamics.cpp:348-363) with e the
-// three field scalars of step i, by the whole block; the zero-padded 64x64 real and
+// three field scalars of step i, by threads tid = 0..nthr-1; the zero-padded 64x64 real and
 // imaginary planes as well when Vpad is not null (instances with N > 32).
 __device__ __forceinline__ void build_v(const InstDev& I, const double2* H, const double2* X, const double2* e,
-                                        double2* V, double* Vpad) {
+                                        double2* V, double* Vpad, int tid, int nthr) {
     const int N = I.N, NN = N * N;
     const double2 e0 = e[0], e1 = e[1], e2 = e[2];
     auto vat = [&](int ix) {
@@ -384,10 +384,10 @@ __device__ __forceinline__ void build_v(const InstDev& I, const double2* H, cons
         v.y -= x2.x * e2.y + x2.y * e2.x;
         return v;
     };
-    for (int q = threadIdx.x; q < NN; q += blockDim.x) V[q] = vat(q);
+    for (int q = tid; q < NN; q += nthr) V[q] = vat(q);
     if (Vpad) {
         double* vi = Vpad + 64 * 64;
-        for (int q = threadIdx.x; q < 64 * 64; q += blockDim.x) {
+        for (int q = tid; q < 64 * 64; q += nthr) {
             const int r = q >> 6, c = q & 63;
             const double2 v = (r < N && c < N) ? vat(r * N + c) : make_double2(0.0, 0.0);
             Vpad[q] = v.x;
@@ -417,11 +417,12 @@ __device__ __forceinline__ void readout_wave(const InstDev& I, const double2* rh
     }
 }
 
-// End of a fused stage, in the workgroup whose tile starts at the instance's ADO 0:
+// End of a fused stage, in the workgroup whose tile starts at the instance's ADO 0
+// (threads tid = 0..nthr-1 of it, wave 0 among them):
 // stage 0 reads out rho_0 before the step (its input, which no stage-0 workgroup
 // writes); stage 3 builds V of the next step into the copy the step after reads
 // (that copy was last read by the previous step's stages, all finished).
-__device__ __forceinline__ void fused_tail(const StageArgs& A, const InstDev& I, int step) {
+__device__ __forceinline__ void fused_tail(const StageArgs& A, const InstDev& I, int step, int tid, int nthr) {
     const FusedDev& F = *A.fz;
     if (A.stage == 0) {
         const int slot = F.pidx[I.step_off + step];
@@ -429,7 +430,8 @@ __device__ __forceinline__ void fused_tail(const StageArgs& A, const InstDev& I,
     } else if (A.stage == 3 && step + 1 < I.nsteps) {
         const long long o = ((step + 1) & 1) ? 1 : 0;
         build_v(I, F.H + I.v_off, F.Xp + 3 * (size_t)I.v_off, F.field + 3 * (size_t)(I.step_off + step + 1),
-                F.V + o * F.vstride + I.v_off, I.vp_off >= 0 ? F.Vpad + o * F.vpstride + I.vp_off : nullptr);
+                F.V + o * F.vstride + I.v_off, I.vp_off >= 0 ? F.Vpad + o * F.vpstride + I.vp_off : nullptr, tid,
+                nthr);
     }
 }
 
@@ -440,7 +442,7 @@ static __global__ __launch_bounds__(256) void prologue(PrologueArgs A) {
     const int step = *A.step_ctr + A.step_off;
     if (step >= I.nsteps) return;
     build_v(I, A.H + I.v_off, A.Xp + 3 * (size_t)I.v_off, A.field + 3 * (size_t)(I.step_off + step), A.V + I.v_off,
-            I.vp_off >= 0 ? A.Vpad + I.vp_off : nullptr);
+            I.vp_off >= 0 ? A.Vpad + I.vp_off : nullptr, threadIdx.x, blockDim.x);
     const int slot = A.pidx[I.step_off + step];
     if (slot < 0 || !A.readout || threadIdx.x >= 64) return;
     readout_wave(I, A.y + (long long)step * A.traj + I.state_off, A.R + I.r_off, slot, A.P + I.p_off);

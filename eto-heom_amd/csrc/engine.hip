@@ -146,7 +146,7 @@ __global__ __launch_bounds__(256) void stage_small(StageArgs A) {
         rk_store(A, I, (size_t)(base + e), st == 0 ? r : yv[q], st == 0 ? make_double2(0.0, 0.0) : av[q], dr, di,
                  a0 + nl, ra, cb);
     }
-    if (A.fused && a0 == 0) fused_tail(A, I, step);
+    if (A.fused && a0 == 0) fused_tail(A, I, step, threadIdx.x, blockDim.x);
 }
 
 // any N <= 64: one ADO per workgroup, rho_n staged in dynamic LDS, V read through the cache
@@ -198,7 +198,7 @@ __global__ __launch_bounds__(256) void stage_generic(StageArgs A) {
         rk_load(A, (size_t)(base + ab), r, yv, av);
         rk_store(A, I, (size_t)(base + ab), yv, av, dr + cpl.x, di + cpl.y, a0, ra, cb);
     }
-    if (A.fused && a0 == 0) fused_tail(A, I, step);
+    if (A.fused && a0 == 0) fused_tail(A, I, step, threadIdx.x, blockDim.x);
 }
 
 // N in (16, NP]: one ADO per workgroup, the commutator on FP64 MFMA 16x16x4 tiles.
@@ -369,7 +369,7 @@ __global__ __launch_bounds__(256, NP == 32 ? (TP ? 3 : 4) : 2) void stage_mfma(S
             }
         }
     }
-    if (A.fused && a0 == 0) fused_tail(A, I, step);
+    if (A.fused && a0 == 0) fused_tail(A, I, step, threadIdx.x, blockDim.x);
 }
 
 // rho_n[r][c] from its N x (N+1) LDS copy, 0 in the padding up to 64 (clamped read, selected)
@@ -503,7 +503,7 @@ __global__ __launch_bounds__(512, 2) void stage_mfma64(StageArgs A) {
             }
         }
     }
-    if (A.fused && a0 == 0) fused_tail(A, I, step);
+    if (A.fused && a0 == 0) fused_tail(A, I, step, threadIdx.x, blockDim.x);
 }
 
 // rho_0 = |0><0| for every instance, all other ADOs zero; step counter to 0
