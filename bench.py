@@ -5,7 +5,7 @@ V(t) built and P(t) read out on the device), with the hierarchy state resident
 in HBM.  The default workload is the heptamer: ground + 7 singles + 21 doubles
 (N=29), 2 ETO modes/site (d=14), depth L=4 (3060 ADOs), 2000 RK4 steps of
 0.5 fs, 3 Gaussian pulses; `value` is ADO-updates/s (one ADO advanced one full
-RK4 step).  Other workloads: --workload dimer | grid2d | chain10 | trimer.
+RK4 step).  Other workloads: --workload dimer | grid2d | chain10 | chain_grid | trimer | trimer_grad.
 
 Printed as one JSON line with the roofline of the dominant kernel (the fused
 RHS+RK4 stage kernel, timed per launch with HIP events on its own stream) and
@@ -38,6 +38,10 @@ def workload(name, nsteps=None):
         return [configs.aggregate(3, 5)], "trimer N=7, d=6, L=5, 462 ADOs, 2000 RK4 steps"
     if name == "chain10":
         return [configs.chain()], "10-site chain N=56, d=20, L=3, 1771 ADOs, 3000 RK4 steps"
+    if name == "chain_grid":
+        return (configs.chain_grid(n_t1=4),
+                "10-site chain N=56, L=3, 1771 ADOs, 3000 RK4 steps: 16 propagations (t1 = 0..30 fs x t2 = "
+                "0/100/200/400 fs) of the 64 x 4 grid in one batch")
     if name == "trimer_grad":
         return None, ("trimer N=7 (ground+3+3), d=6, L=5, 462 ADOs: 16x16 (t1, t3) rephasing grid at t2=100 fs, "
                       "48 propagations, loss gradient by the adjoint (checkpoint every 32 steps)")

@@ -143,6 +143,12 @@ def chain(n_sites=10, L=3, nsteps=3000, dt=0.5, seed=10, t1=50.0, t2=100.0):
     return realise(txt)[0]
 
 
+def chain_grid(n_t1=64, t2s=(0.0, 100.0, 200.0, 400.0), nsteps=3000):
+    """The chain's batched (t1, t2) grid: t1 = 0, 10, ... fs (n_t1 values) x t2s, one
+    propagation each (64 x 4 = 256 for the full grid, about 91 GB of state)."""
+    return [chain(t1=10.0 * i, t2=t2, nsteps=nsteps) for t2 in t2s for i in range(n_t1)]
+
+
 def aggregate(n_sites, L, eps0=12000.0, J=100.0, seed=1, sigma=0.0, taus=(40.0, 90.0, 190.0), nsteps=2000,
               dt=0.5, sample=0):
     """n_sites in a ring-free chain with coupling J, optional static disorder sigma (cm^-1)."""

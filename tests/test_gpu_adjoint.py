@@ -148,3 +148,28 @@ def test_adjoint_generic_large_n(ctx):
         ref = fd(p, w, name, idx, part)
         got = adj_value(g, name, idx, part)
         assert abs(got - ref) / max(abs(ref), 1e-12) < 1e-6, (name, idx, got, ref)
+
+
+def test_adjoint_chain_n56(ctx):
+    """N = 56, the 10-site chain's system (32 < N <= 64: the stage_mfma64 forward, the
+    generic adjoint kernel): site energies, a nearest-neighbour coupling, a
+    double-exciton entry and a dipole entry against FD."""
+    p = configs.chain(L=1, nsteps=24)
+    p.pulses[:, 1] = [2.0, 5.0, 8.0]
+    p.pulses[:, 2] = 3.0
+    p.pulses[:, 0] *= 3e5
+    assert p.N == 56
+    P0, _ = O.twod(oproblem(p), nv=(1, 1, 1))
+    rng = np.random.default_rng(4)
+    w = rng.normal(size=len(P0)) + 1j * rng.normal(size=len(P0))
+    b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)}], adjoint=8)
+    b.run()
+    b.sync()
+    assert np.max(np.abs(b.output(0) - P0)) / np.max(np.abs(P0)) < 1e-10
+    b.backward([w])
+    g = b.grad(0)
+    for name, idx, part in [("H", (1, 1), "re"), ("H", (6, 6), "re"), ("H", (1, 2), "re"), ("H", (11, 12), "re"),
+                            ("X", (0, 0, 4), "re")]:
+        ref = fd(p, w, name, idx, part)
+        got = adj_value(g, name, idx, part)
+        assert abs(got - ref) / max(abs(ref), 1e-12) < 1e-6, (name, idx, got, ref)

@@ -7,7 +7,9 @@
   the plain per-launch path and to a second copy of the instance in the same batch;
 * the 512-propagation grid: sampled instances of the full batch bitwise equal to the
   same instances solved alone (eh_2des_solve), traces conserved;
-* the 10-site chain N=56, L=3 (1771 ADOs, 3000 steps): trace conserved, finite.
+* the 10-site chain N=56, L=3 (1771 ADOs, 3000 steps): trace conserved, finite; four
+  propagations of its (t1, t2) grid in one batch bitwise equal to the same propagations
+  solved alone.
 """
 import numpy as np
 import pytest
@@ -84,3 +86,16 @@ def test_chain_n56_full_run_conserves_trace(ctx):
     P, rho = E.twoD_spectrum_solver(ctx, p, p.H, 1, 1, 1, X=p.X, want_rho=True)
     assert np.all(np.isfinite(P)) and np.max(np.abs(P)) > 0
     assert abs(np.trace(rho[0]) - 1.0) < 1e-12
+
+
+def test_chain_grid_batch_equals_propagations_solved_alone(ctx):
+    items = configs.chain_grid(n_t1=2, t2s=(0.0, 400.0))    # 4 of the grid's propagations, full size
+    b = E.Batch(ctx, [item(p) for p in items])
+    b.run()
+    b.sync()
+    for i in (0, 3):
+        p = items[i]
+        P, rho = E.twoD_spectrum_solver(ctx, p, p.H, 1, 1, 1, X=p.X, want_rho=True)
+        assert np.array_equal(b.output(i), P), i
+        assert np.array_equal(b.rho(i), rho), i
+    assert not np.array_equal(b.output(0), b.output(3))
