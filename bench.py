@@ -137,7 +137,7 @@ def bench_gradient(a):
     with torch.no_grad():
         target = SP.grid_signal(SP.trimer_params(perturb=5), spec)
     import gc
-    fw, bw, gfw, gbw = [], [], [], []
+    fw, bw, gfw, gbw, gpf = [], [], [], [], []
     for it in range(a.warmup + a.steps):
         params = SP.trimer_params()
         torch.cuda.synchronize()
@@ -155,9 +155,13 @@ def bench_gradient(a):
         lval = float(loss.detach())
         del S, loss   # the graph holds the batch (and its trajectory): free it before the next forward
         gc.collect()
+        with torch.no_grad():   # one plain forward solve (no adjoint state) of the same grid
+            SP.grid_signal(params, spec)
+        if it >= a.warmup:
+            gpf.append(spec.timing["forward_ms"] * 1e-3)
     n_params = sum(p.numel() for p in params.values())
     f, b = float(np.mean(fw)), float(np.mean(bw))
-    gf, gb = float(np.mean(gfw)), float(np.mean(gbw))
+    gf, gb, gp = float(np.mean(gfw)), float(np.mean(gbw)), float(np.mean(gpf))
     # ADO-updates of the forward: 48 propagations x 462 ADOs x their steps
     steps = sum(int(np.floor(np.float32(np.float32(te) / np.float32(spec.step_size)))) + 1 for te in spec.t_end)
     ups = 462 * steps
@@ -168,12 +172,16 @@ def bench_gradient(a):
            "config": {"workload": workload("trimer_grad")[1], "device": ctx.name,
                       "grid_points_per_s": 16 / gf,
                       "adjoint": {"forward_gpu_s": gf, "backward_gpu_s": gb, "gradient_over_forward": (gf + gb) / gf,
+                                  "plain_forward_gpu_s": gp, "gradient_over_plain_forward": (gf + gb) / gp,
                                   "forward_wall_s": f, "backward_wall_s": b,
                                   "gradient_over_forward_wall": (f + b) / f,
                                   "n_params": n_params, "central_fd_s": 2 * n_params * gf,
                                   "fd_over_adjoint": 2 * n_params * gf / (gf + gb), "loss": lval,
-                                  "note": "GPU: HIP events around the propagation and the backward pass; wall: "
-                                          "incl. host set-up of each batch (tables, graph capture, allocation)"}}}
+                                  "note": "GPU: HIP events around the propagation and the backward pass; "
+                                          "forward = the gradient run's forward (it also keeps the adjoint's "
+                                          "trajectory), plain forward = the same grid with no gradient wanted; "
+                                          "wall: incl. host set-up of each batch (tables, graph capture, "
+                                          "allocation)"}}}
     print(json.dumps(rec))
     return 0
 

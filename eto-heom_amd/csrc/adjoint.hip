@@ -70,18 +70,21 @@ __device__ __forceinline__ double2 cconj_mul(double2 a, double2 b) {   // conj(a
     return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
 }
 
-// adjoint RK stage update of one element (coefficients of the reverse step above)
+// adjoint RK stage update of one element (coefficients of the reverse step above).
+// Stage 0 writes no accumulator: its output is x = h/3 lam + h^2/6 u0, so stage 1
+// rebuilds lam + h/6 u0 = 2/3 lam + x / h from its own input x, passed as `acc`.
 __device__ __forceinline__ void rk_store_adj(const StageArgs& A, const InstDev& I, size_t gi, double2 lam, double2 acc,
                                              double ur, double ui) {
     const int st = A.stage;
     const double h = I.h;
     if (st == 0) {
-        const double a = h / 6.0, b = h / 3.0, c = h * h / 6.0;
-        A.acc[gi] = make_double2(fma(a, ur, lam.x), fma(a, ui, lam.y));
+        const double b = h / 3.0, c = h * h / 6.0;
         A.out[gi] = make_double2(fma(c, ur, b * lam.x), fma(c, ui, b * lam.y));
     } else if (st < 3) {
         const double b = st == 1 ? h / 3.0 : h / 6.0, c = h / 2.0;
-        A.acc[gi] = make_double2(acc.x + ur, acc.y + ui);
+        const double ih = 1.0 / h, t = 2.0 / 3.0;
+        const double2 a = st == 1 ? make_double2(fma(t, lam.x, acc.x * ih), fma(t, lam.y, acc.y * ih)) : acc;
+        A.acc[gi] = make_double2(a.x + ur, a.y + ui);
         A.out[gi] = make_double2(fma(c, ur, b * lam.x), fma(c, ui, b * lam.y));
     } else {
         A.out[gi] = make_double2(acc.x + ur, acc.y + ui);
@@ -261,7 +264,7 @@ __global__ __launch_bounds__(256) void adj_small(AdjArgs J) {
     const int dq = na * I.d;
     const int np_pre = dq > 0 ? J.ado[I.lk_off + (long long)a0 * I.d + (threadIdx.x < dq ? threadIdx.x : 0)] : 0;
     const double2* lp = (st == 1 || st == 2) ? A.y : A.rin;
-    const double2* ap = st > 0 ? A.acc : A.rin;
+    const double2* ap = st >= 2 ? A.acc : A.rin;   // (stage 1 passes its input as rk_store_adj's acc)
     double2 rv[EPT], lv[EPT], av[EPT], yf[EPT], gv[EPT];
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
@@ -486,7 +489,7 @@ __global__ __launch_bounds__(256) void adj_generic(AdjArgs J) {
         ui = fma(g.x, r.y, fma(g.y, r.x, ui));
         const double2 cpl = coupling_of(I, sC, ra, cb);
         const double2 lam = st == 0 ? r : (st < 3 ? A.y[base + ab] : make_double2(0.0, 0.0));
-        const double2 av = st > 0 ? A.acc[base + ab] : make_double2(0.0, 0.0);
+        const double2 av = st >= 2 ? A.acc[base + ab] : r;
         rk_store_adj(A, I, (size_t)(base + ab), lam, av, ur + cpl.x, ui + cpl.y);
     }
     red[threadIdx.x] = qsum;
@@ -510,31 +513,13 @@ __global__ __launch_bounds__(256) void adj_generic(AdjArgs J) {
     }
 }
 
-struct AdjPrologueArgs {
+struct AdjBetweenArgs {
     const InstDev* inst;
     const int* step_ctr;
+    int readout;           // the readout terms of step `step_off`
     int step_off;
-    const double2* V;   // forward V_i, written by the recompute's prologue
-    double2* VH;
-};
-
-// V_i^H = conj(V_i)^T for the adjoint stages of step i
-__global__ __launch_bounds__(256) void adj_prologue(AdjPrologueArgs P) {
-    const InstDev& I = P.inst[blockIdx.x];
-    const int step = *P.step_ctr + P.step_off;
-    if (step < 0 || step >= I.nsteps) return;
-    const int N = I.N, NN = N * N;
-    for (int q = threadIdx.x; q < NN; q += blockDim.x) {
-        const int a = q / N, b = q - a * N;
-        const double2 v = P.V[I.v_off + b * N + a];
-        P.VH[I.v_off + q] = make_double2(v.x, -v.y);
-    }
-}
-
-struct AdjReadoutArgs {
-    const InstDev* inst;
-    const int* step_ctr;
-    int step_off;
+    int vh;                // V^H of step `vh_step_off`
+    int vh_step_off;
     const int* pidx;
     const double2* R;      // readout operators (X3)
     const double2* gP;     // dL/dP per readout
@@ -543,31 +528,55 @@ struct AdjReadoutArgs {
     double2* gacc;
     const long long* gacc_off;
     long long traj;        // trajectory mode: y moves by step * traj elements
+    const double2* H;      // V of a step from H, X'_q and the field (as the forward builds it)
+    const double2* Xp;
+    const double2* field;
+    double2* VH;
 };
 
-// l[ADO 0][j][k] += conj(X3[k][j]) g_i ;  dL/dX3[k][j] += conj(rho_0[j][k]) g_i
-__global__ __launch_bounds__(256) void adj_readout(AdjReadoutArgs P) {
+// Between the adjoint stages of step i and those of step i-1, one workgroup per
+// instance: the readout terms of step i
+//   l[ADO 0][j][k] += conj(X3[k][j]) g_i ;  dL/dX3[k][j] += conj(rho_0[j][k]) g_i
+// and V_{i-1}^H = conj(V_{i-1})^T for the next step's adjoint stages (V built from H,
+// X'_q and the field exactly as the forward builds it).  The two touch different data.
+__global__ __launch_bounds__(256) void adj_between(AdjBetweenArgs P) {
     const int ii = blockIdx.x;
     const InstDev& I = P.inst[ii];
-    const int step = *P.step_ctr + P.step_off;
-    if (step < 0 || step >= I.nsteps || I.nread != 1) return;
-    const int slot = P.pidx[I.step_off + step];
-    if (slot < 0) return;
-    const double2 g = P.gP[I.p_off + slot];
     const int N = I.N, NN = N * N;
-    double2* acc = P.gacc + P.gacc_off[ii] + 4 * NN;
-    for (int q = threadIdx.x; q < NN; q += blockDim.x) {
-        const int j = q / N, k = q - j * N;
-        const double2 x = P.R[I.r_off + k * N + j];
-        double2 l = P.lam[I.state_off + q];
-        const double2 t = cconj_mul(x, g);
-        l.x += t.x;
-        l.y += t.y;
-        P.lam[I.state_off + q] = l;
-        const double2 r = P.y[(long long)step * P.traj + I.state_off + q];
-        const double2 u = cconj_mul(r, g);
-        acc[k * N + j].x += u.x;
-        acc[k * N + j].y += u.y;
+    if (P.readout) {
+        const int step = *P.step_ctr + P.step_off;
+        const int slot = (step >= 0 && step < I.nsteps && I.nread == 1) ? P.pidx[I.step_off + step] : -1;
+        if (slot >= 0) {
+            const double2 g = P.gP[I.p_off + slot];
+            double2* acc = P.gacc + P.gacc_off[ii] + 4 * NN;
+            for (int q = threadIdx.x; q < NN; q += blockDim.x) {
+                const int j = q / N, k = q - j * N;
+                const double2 x = P.R[I.r_off + k * N + j];
+                double2 l = P.lam[I.state_off + q];
+                const double2 t = cconj_mul(x, g);
+                l.x += t.x;
+                l.y += t.y;
+                P.lam[I.state_off + q] = l;
+                const double2 r = P.y[(long long)step * P.traj + I.state_off + q];
+                const double2 u = cconj_mul(r, g);
+                acc[k * N + j].x += u.x;
+                acc[k * N + j].y += u.y;
+            }
+        }
+    }
+    if (P.vh) {
+        const int step = *P.step_ctr + P.vh_step_off;
+        if (step >= 0 && step < I.nsteps) {
+            const double2* e = P.field + 3 * (size_t)(I.step_off + step);
+            const double2 e0 = e[0], e1 = e[1], e2 = e[2];
+            const double2* H = P.H + I.v_off;
+            const double2* X = P.Xp + 3 * (size_t)I.v_off;
+            for (int q = threadIdx.x; q < NN; q += blockDim.x) {
+                const int a = q / N, b = q - a * N;
+                const double2 v = v_elem(H, X, NN, e0, e1, e2, b * N + a);
+                P.VH[I.v_off + q] = make_double2(v.x, -v.y);
+            }
+        }
     }
 }
 
@@ -755,13 +764,13 @@ int batch_enable_adjoint(Batch* b, int C) {
         pa.y = tslot(0);
         pa.traj = 4 * (long long)T;
     }
-    // forward: C steps + (checkpoint store) + tick
+    // forward: C fused steps (readouts in stage 0, the next step's V in stage 3, as the
+    // forward graph does) + (checkpoint store) + tick
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < a->C; k++) {
-        pa.step_off = k;
-        hipLaunchKernelGGL(prologue, dim3(b->n_inst), dim3(256), 0, s, pa);
         for (int st = 0; st < 4; st++) {
             StageArgs x = stage_io(fw, im, k, st);
+            x.fused = 1;
             if (a->trajm) {   // Y_i, S1, S2, S1' of step i in slots 4i .. 4i+3; Y_{i+1} in 4i+4
                 x.y = tslot(0);
                 x.rin = tslot(st);
@@ -808,12 +817,8 @@ int batch_enable_adjoint(Batch* b, int C) {
     aj.ncopy = a->ncopy;
     aj.slots = a->slots.p;
     aj.slot_off = a->slot_off.p;
-    AdjPrologueArgs ap;
-    ap.inst = im->inst.p;
-    ap.step_ctr = im->step_ctr.p;
-    ap.V = im->V.p;
-    ap.VH = a->VH.p;
-    AdjReadoutArgs ar;
+    AdjBetweenArgs ar;
+    std::memset(&ar, 0, sizeof(ar));
     ar.inst = im->inst.p;
     ar.step_ctr = im->step_ctr.p;
     ar.pidx = im->pidx.p;
@@ -823,6 +828,10 @@ int batch_enable_adjoint(Batch* b, int C) {
     ar.gacc = a->gacc.p;
     ar.gacc_off = a->gacc_off.p;
     ar.traj = a->trajm ? 4 * (long long)T : 0;
+    ar.H = im->H.p;
+    ar.Xp = im->Xp.p;
+    ar.field = im->field.p;
+    ar.VH = a->VH.p;
     aj.traj = a->trajm ? 4 * (long long)T : 0;
     pa.readout = 0;   // replays must not overwrite the forward readouts
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -848,11 +857,11 @@ int batch_enable_adjoint(Batch* b, int C) {
                 }
         }
     }
+    ar.readout = 0;   // V^H of the segment's last step
+    ar.vh = 1;
+    ar.vh_step_off = a->C - 1;
+    hipLaunchKernelGGL(adj_between, dim3(b->n_inst), dim3(256), 0, s, ar);
     for (int k = a->C - 1; k >= 0; k--) {
-        pa.step_off = k;   // V_i again (the replay left V at the segment's last step)
-        hipLaunchKernelGGL(prologue, dim3(b->n_inst), dim3(256), 0, s, pa);
-        ap.step_off = k;
-        hipLaunchKernelGGL(adj_prologue, dim3(b->n_inst), dim3(256), 0, s, ap);
         for (int st = 0; st < 4; st++) {
             AdjArgs x = aj;
             x.S.step_off = k;
@@ -869,9 +878,12 @@ int batch_enable_adjoint(Batch* b, int C) {
                     return rc;
                 }
         }
+        ar.readout = 1;   // step k's readout terms, and V^H of step k-1
         ar.step_off = k;
         ar.y = a->trajm ? tslot(0) : stg(k, 0);
-        hipLaunchKernelGGL(adj_readout, dim3(b->n_inst), dim3(256), 0, s, ar);
+        ar.vh = k > 0;
+        ar.vh_step_off = k - 1;
+        hipLaunchKernelGGL(adj_between, dim3(b->n_inst), dim3(256), 0, s, ar);
     }
     hipLaunchKernelGGL(tick, dim3(1), dim3(1), 0, s, im->step_ctr.p, -a->C);
     HIPCHK(hipStreamEndCapture(s, &a->gbwd));

@@ -106,9 +106,13 @@ __device__ __forceinline__ double stage_hb(const InstDev& I, int st) {
     return st == 0 ? I.hb[0] : (st == 1 ? I.hb[1] : (st == 2 ? I.hb[2] : I.hb[3]));
 }
 
-// RK4 stage update of one element from its RHS value k, with y / acc already loaded:
-//   stage 0: ACC = y + h/6 k, out = y + h/2 k;  1: ACC += h/3 k, out = y + h/2 k;
-//   2: ACC += h/3 k, out = y + h k;  3: out (= Y) = ACC + h/6 k
+// RK4 stage update of one element from its RHS value k (the reference's classic RK4,
+// dynamics.cpp:386-417, in 8 reads + 6 writes of the state per step instead of 9 + 7):
+//   stage 0: out = y + h/2 k                       (no ACC write)
+//   stage 1: ACC = y + (x - y)/3 + h/3 k, out = y + h/2 k, where x = y + h/2 k1 is this
+//            stage's input, passed as `a` (so y + (x - y)/3 = y + h/6 k1: ACC is rebuilt
+//            here instead of written by stage 0 and read back)
+//   stage 2: ACC += h/3 k, out = y + h k;  stage 3: out (= Y) = ACC + h/6 k
 // Element (row, col) of ADO m (instance-local index) also goes to the column copy.
 __device__ __forceinline__ void rk_store(const StageArgs& A, const InstDev& I, size_t gi, double2 y, double2 a,
                                          double kr, double ki, int m, int row, int col) {
@@ -116,11 +120,11 @@ __device__ __forceinline__ void rk_store(const StageArgs& A, const InstDev& I, s
     const double hb = stage_hb(I, st);
     double2 o;
     if (st == 0) {
-        A.acc[gi] = make_double2(fma(hb, kr, y.x), fma(hb, ki, y.y));
         o = make_double2(fma(I.h2, kr, y.x), fma(I.h2, ki, y.y));
     } else if (st < 3) {
         const double hs = st == 1 ? I.h2 : I.h;
-        A.acc[gi] = make_double2(fma(hb, kr, a.x), fma(hb, ki, a.y));
+        const double2 b = st == 1 ? make_double2(fma(a.x - y.x, 1.0 / 3.0, y.x), fma(a.y - y.y, 1.0 / 3.0, y.y)) : a;
+        A.acc[gi] = make_double2(fma(hb, kr, b.x), fma(hb, ki, b.y));
         o = make_double2(fma(hs, kr, y.x), fma(hs, ki, y.y));
     } else {
         o = make_double2(fma(hb, kr, a.x), fma(hb, ki, a.y));
@@ -142,21 +146,12 @@ __device__ __forceinline__ const double2* nb_elem(const StageArgs& A, const Inst
     return (A.rin + I.state_off) + (unsigned)(mu * N * N + (rc == 0 ? s * N + pos : pos * N + s));
 }
 
-// y and acc of element gi as the stage needs them (stage 0 takes y = rin value)
+// y and the `a` operand of rk_store for element gi as the stage needs them (stage 0
+// takes y = rin value; stage 1 takes a = rin value)
 __device__ __forceinline__ void rk_load(const StageArgs& A, size_t gi, double2 rin_val, double2& y, double2& a) {
     const int st = A.stage;
     y = st == 0 ? rin_val : (st < 3 ? A.y[gi] : make_double2(0.0, 0.0));
-    a = st == 0 ? make_double2(0.0, 0.0) : A.acc[gi];
-}
-
-// y and acc of element gi as the stage needs them, without touching the stage
-// input (stage 0 takes y = rin value at the store: no wait on that load here)
-__device__ __forceinline__ void rk_load_ya(const StageArgs& A, size_t gi, double2& y, double2& a) {
-    const int st = A.stage;
-    y = make_double2(0.0, 0.0);
-    a = make_double2(0.0, 0.0);
-    if (st == 1 || st == 2) y = A.y[gi];
-    if (st != 0) a = A.acc[gi];
+    a = st >= 2 ? A.acc[gi] : rin_val;
 }
 
 // Coupling gathers of one ADO into LDS.  For bath j (site s = j + so), the
@@ -369,21 +364,25 @@ struct PrologueArgs {
 // V(t_i) = H - e_1 X[0] - e_2 X[1]^T - e_3 X[2]^T  (dynamics.cpp:348-363) with e the
 // three field scalars of step i, by threads tid = 0..nthr-1; the zero-padded 64x64 real and
 // imaginary planes as well when Vpad is not null (instances with N > 32).
+// element ix of V = H - e_1 X[0] - e_2 X[1]^T - e_3 X[2]^T (X holds the three N*N blocks)
+__device__ __forceinline__ double2 v_elem(const double2* H, const double2* X, int NN, double2 e0, double2 e1,
+                                          double2 e2, int ix) {
+    double2 v = H[ix];
+    const double2 x0 = X[ix], x1 = X[NN + ix], x2 = X[2 * NN + ix];
+    v.x -= x0.x * e0.x - x0.y * e0.y;
+    v.y -= x0.x * e0.y + x0.y * e0.x;
+    v.x -= x1.x * e1.x - x1.y * e1.y;
+    v.y -= x1.x * e1.y + x1.y * e1.x;
+    v.x -= x2.x * e2.x - x2.y * e2.y;
+    v.y -= x2.x * e2.y + x2.y * e2.x;
+    return v;
+}
+
 __device__ __forceinline__ void build_v(const InstDev& I, const double2* H, const double2* X, const double2* e,
                                         double2* V, double* Vpad, int tid, int nthr) {
     const int N = I.N, NN = N * N;
     const double2 e0 = e[0], e1 = e[1], e2 = e[2];
-    auto vat = [&](int ix) {
-        double2 v = H[ix];
-        const double2 x0 = X[ix], x1 = X[NN + ix], x2 = X[2 * NN + ix];
-        v.x -= x0.x * e0.x - x0.y * e0.y;
-        v.y -= x0.x * e0.y + x0.y * e0.x;
-        v.x -= x1.x * e1.x - x1.y * e1.y;
-        v.y -= x1.x * e1.y + x1.y * e1.x;
-        v.x -= x2.x * e2.x - x2.y * e2.y;
-        v.y -= x2.x * e2.y + x2.y * e2.x;
-        return v;
-    };
+    auto vat = [&](int ix) { return v_elem(H, X, NN, e0, e1, e2, ix); };
     for (int q = tid; q < NN; q += nthr) V[q] = vat(q);
     if (Vpad) {
         double* vi = Vpad + 64 * 64;

@@ -71,9 +71,10 @@ __global__ __launch_bounds__(256) void stage_small(StageArgs A) {
     if (has_cpl) gather_idx<2, 2, NT>(A, I, a0, na, threadIdx.x, 256, nbx);
     const bool vreal = A.vreal[I.step_off + step] != 0;
     // y and acc where the stage reads them, else the (just loaded, cached) input
+    // (which is what stage 1 passes as rk_store's `a`)
     const int st = A.stage;
     const double2* yp = (st == 1 || st == 2) ? A.y : A.rin;
-    const double2* ap = st != 0 ? A.acc : A.rin;
+    const double2* ap = st >= 2 ? A.acc : A.rin;
     double2 rv[EPT], yv[EPT], av[EPT], gv[EPT];
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
@@ -249,7 +250,7 @@ __global__ __launch_bounds__(256, NP == 32 ? (TP ? 3 : 4) : 2) void stage_mfma(S
     double2 yv[TPW][4], av[TPW][4];
     if (TP) {   // every load unconditional: clamped address, selected result
         const double2* yp = (A.stage == 1 || A.stage == 2) ? A.y : A.rin;
-        const double2* ap = A.stage != 0 ? A.acc : A.rin;
+        const double2* ap = A.stage >= 2 ? A.acc : A.rin;
 #pragma unroll
         for (int q = 0; q < SPT; ++q) {
             const int e = threadIdx.x + q * 256;
@@ -297,7 +298,7 @@ __global__ __launch_bounds__(256, NP == 32 ? (TP ? 3 : 4) : 2) void stage_mfma(S
                 if (row < N && col < N && A.stage != 0) {
                     const size_t gi = (size_t)(base + row * N + col);
                     if (A.stage < 3) yv[t][r] = A.y[gi];
-                    av[t][r] = A.acc[gi];
+                    if (A.stage >= 2) av[t][r] = A.acc[gi];
                 }
             }
         }
@@ -365,7 +366,8 @@ __global__ __launch_bounds__(256, NP == 32 ? (TP ? 3 : 4) : 2) void stage_mfma(S
                 di = fma(g.x, x.y, fma(g.y, x.x, di));
                 const double2 cpl = coupling_of(I, sC, row, col);
                 const double2 y0 = A.stage == 0 ? x : yv[t][r];
-                rk_store(A, I, (size_t)(base + row * N + col), y0, av[t][r], dr + cpl.x, di + cpl.y, a0, row, col);
+                const double2 a1 = A.stage == 1 ? x : av[t][r];
+                rk_store(A, I, (size_t)(base + row * N + col), y0, a1, dr + cpl.x, di + cpl.y, a0, row, col);
             }
         }
     }
@@ -472,7 +474,7 @@ __global__ __launch_bounds__(512, 2) void stage_mfma64(StageArgs A) {
     }
     // y and acc of the lane's 8 output elements (unconditional, clamped), then the update
     const double2* yp = (st == 1 || st == 2) ? A.y : A.rin;
-    const double2* ap = st != 0 ? A.acc : A.rin;
+    const double2* ap = st >= 2 ? A.acc : A.rin;
     double2 yv[2][4], av[2][4];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
@@ -613,7 +615,7 @@ __device__ __forceinline__ void load_dat(const StageArgs& A, const Tile& T, cons
         if (row < N && col < N && A.stage != 0) {
             const size_t gi = (size_t)(base + row * N + col);
             if (A.stage < 3) D.y[r] = A.y[gi];
-            D.a[r] = A.acc[gi];
+            if (A.stage >= 2) D.a[r] = A.acc[gi];
         }
     }
     const int per = 2 * T.K * N;
@@ -794,12 +796,16 @@ __global__ __launch_bounds__(256, 2) void stage_mfma_pipe(StageArgs A, int ntile
                     }
                     const size_t gi = (size_t)(base + row * N + col);
                     if (st == 0) {
-                        A.acc[gi] = make_double2(fma(hb, kr, x.x), fma(hb, ki, x.y));
                         A.out[gi] = make_double2(fma(h2, kr, x.x), fma(h2, ki, x.y));
                     } else if (st < 3) {
+                        // stage 1 rebuilds ACC = y + h/6 k1 from its input x = y + h/2 k1
                         const double hs = st == 1 ? h2 : h;
-                        A.acc[gi] = make_double2(fma(hb, kr, acur[r].x), fma(hb, ki, acur[r].y));
-                        A.out[gi] = make_double2(fma(hs, kr, ycur[r].x), fma(hs, ki, ycur[r].y));
+                        const double2 yc = ycur[r];
+                        const double2 b = st == 1 ? make_double2(fma(x.x - yc.x, 1.0 / 3.0, yc.x),
+                                                                 fma(x.y - yc.y, 1.0 / 3.0, yc.y))
+                                                  : acur[r];
+                        A.acc[gi] = make_double2(fma(hb, kr, b.x), fma(hb, ki, b.y));
+                        A.out[gi] = make_double2(fma(hs, kr, yc.x), fma(hs, ki, yc.y));
                     } else {
                         A.out[gi] = make_double2(fma(hb, kr, acur[r].x), fma(hb, ki, acur[r].y));
                     }

@@ -45,7 +45,7 @@ class Spec:
 
 class _Propagate(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, H, X, alpha, alpha_t, gamma, spec: Spec):
+    def forward(ctx, H, X, alpha, alpha_t, gamma, spec: Spec, want_grad: bool):
         B = H.shape[0]
         items = []
         for i in range(B):
@@ -59,7 +59,9 @@ class _Propagate(torch.autograd.Function):
                         step_size=spec.step_size, print_step=spec.print_step)
             nv = spec.nv[i] if spec.nv else (1, 1, 1)
             items.append({"problem": p, "H": p.H, "X": p.X, "nv": tuple(nv), "mode": MODE_2DES})
-        b = Batch(spec.ctx, items, adjoint=spec.checkpoint)
+        # the adjoint's state (trajectory or checkpoints) only when a gradient is wanted
+        need = bool(want_grad)
+        b = Batch(spec.ctx, items, adjoint=spec.checkpoint if need else 0)
         b.run()
         b.sync()
         outs = [b.output(i) for i in range(B)]
@@ -67,15 +69,19 @@ class _Propagate(torch.autograd.Function):
         P = np.zeros((B, max(lens) if lens else 0), dtype=np.complex128)
         for i, o in enumerate(outs):
             P[i, : len(o)] = o
-        ctx.batch = b
+        ctx.batch = b if need else None
         ctx.lens = lens
         ctx.spec = spec
         spec.timing["forward_ms"] = b.elapsed_ms()
+        spec.timing["forward_adjoint"] = need
         return torch.from_numpy(P)
 
     @staticmethod
     def backward(ctx, gP):
         b = ctx.batch
+        if b is None:
+            raise RuntimeError("propagate: the forward ran without gradients (grad mode off or no input "
+                               "requiring grad), so there is no adjoint state")
         g = gP.detach().cpu().numpy().astype(np.complex128)
         b.backward([g[i, :n] for i, n in enumerate(ctx.lens)])
         ctx.spec.timing["backward_ms"] = b.elapsed_ms()
@@ -85,12 +91,14 @@ class _Propagate(torch.autograd.Function):
         def stack(name):
             return torch.from_numpy(np.stack([gr[name] for gr in grads]))
 
-        return stack("H"), stack("X"), stack("alpha"), stack("alpha_t"), stack("gamma"), None
+        return stack("H"), stack("X"), stack("alpha"), stack("alpha_t"), stack("gamma"), None, None
 
 
 def propagate(H, X, alpha, alpha_t, gamma, spec: Spec):
-    """P[b, t] of a batch of 2DES propagations, differentiable in all five inputs."""
-    return _Propagate.apply(H, X, alpha, alpha_t, gamma, spec)
+    """P[b, t] of a batch of 2DES propagations, differentiable in all five inputs.
+    The adjoint's state is kept only when grad mode is on and an input requires grad."""
+    want = torch.is_grad_enabled() and any(t.requires_grad for t in (H, X, alpha, alpha_t, gamma))
+    return _Propagate.apply(H, X, alpha, alpha_t, gamma, spec, want)
 
 
 def build_2des_system(eps_cm, J_cm, mu, lambda_line, polar):
