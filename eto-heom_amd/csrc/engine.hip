@@ -6,21 +6,24 @@
 //
 //  * Every instance (one propagation) owns a contiguous slice [M][N][N] of four
 //    complex128 state buffers: Y (the state), ACC (the RK4 accumulator) and two
-//    stage-input buffers S1/S2.  A step is a prologue kernel (V(t_i) and the
-//    readout of Tr(rho_0 R) before the step) and four fused RHS+RK4 stage kernels:
-//      stage 0: k1 = f(Y)   ACC = Y + h/6 k1     S1 = Y + h/2 k1
-//      stage 1: k2 = f(S1)  ACC += h/3 k2        S2 = Y + h/2 k2
+//    stage-input buffers S1/S2.  A step is four fused RHS+RK4 stage kernels:
+//      stage 0: k1 = f(Y)   S1 = Y + h/2 k1
+//      stage 1: k2 = f(S1)  ACC = Y + (S1 - Y)/3 + h/3 k2   S2 = Y + h/2 k2
 //      stage 2: k3 = f(S2)  ACC += h/3 k3        S1 = Y + h k3
 //      stage 3: k4 = f(S1)  Y = ACC + h/6 k4
-//    which is the reference's in-place RK4 (dynamics.cpp:385-417) with V frozen
-//    over the step, and 9 reads + 7 writes of each ADO per step.
+//    which is the reference's RK4 (dynamics.cpp:385-417) with V frozen over the
+//    step (stage 1 rebuilds Y + h/6 k1 from its input instead of stage 0 writing
+//    it: 8 reads + 6 writes of each ADO per step).  In the forward graph stage 0
+//    also reads out Tr(rho_0 R) before the step and stage 3 builds V of the next
+//    step (double-buffered by step parity); elsewhere a prologue kernel does both.
 //  * f(rho)_n[a][b] = -i (V rho_n - rho_n V)[a][b] + damp_n rho_n[a][b]
 //                     + sum_links ([a==s] w_row + [b==s] w_col) rho_m[a][b]
 //    i.e. the one-hot S_j products of the reference reduce to element gathers of
 //    the same (a, b) entry of the neighbour ADO.
-//  * Small N (<= 16): one workgroup per tile of G = 512/N^2 ADOs, the tile and V
-//    staged in LDS, one thread per matrix element.  Larger N: one ADO per
-//    workgroup with the commutator from LDS (the MFMA path replaces it).
+//  * Small N (<= 16): one workgroup per tile of G = GE/N^2 ADOs, the tile and V
+//    staged in LDS, GE/256 matrix elements per thread.  Larger N: one ADO per
+//    workgroup with the commutator on FP64 MFMA tiles (N <= 32: 256 threads;
+//    N <= 64: 512 threads), or from LDS on the VALU (stage_generic).
 //  * The time loop is captured as a HIP graph of `chunk` steps; kernels read the
 //    step index from a device counter that a one-thread kernel advances, so one
 //    instantiated graph is replayed for the whole propagation.
