@@ -320,6 +320,7 @@ __global__ __launch_bounds__(256, NP == 32 ? (TP ? 3 : 4) : 2) void stage_mfma(S
     __syncthreads();
     const bool vreal = A.vreal[I.step_off + step] != 0;
     const double2 g = A.damp[I.ado_off + a0];
+    const int kmax = (N + 3) & ~3;   // contraction over k < N, rounded up to the MFMA's 4
     auto vload = [&](int r, int c) -> double2 {
         if (VLDS) return make_double2(sVr[r * LD + c], sVi[r * LD + c]);
         return (r < N && c < N) ? V[r * N + c] : make_double2(0.0, 0.0);
@@ -334,6 +335,7 @@ __global__ __launch_bounds__(256, NP == 32 ? (TP ? 3 : 4) : 2) void stage_mfma(S
         if (vreal) {
 #pragma unroll 4
             for (int k0 = 0; k0 < NP; k0 += 4) {
+                if (NP == 32 && k0 >= kmax) break;   // the padding beyond N holds zeros
                 const int kk = k0 + lk;
                 const double va = vload(ar, kk).x, vb = vload(kk, bc).x;
                 const double2 rb = sR[kk * LD + bc], ra = sR[ar * LD + kk];
@@ -345,6 +347,7 @@ __global__ __launch_bounds__(256, NP == 32 ? (TP ? 3 : 4) : 2) void stage_mfma(S
         } else {
 #pragma unroll 2
             for (int k0 = 0; k0 < NP; k0 += 4) {
+                if (NP == 32 && k0 >= kmax) break;
                 const int kk = k0 + lk;
                 const double2 va = vload(ar, kk), vb = vload(kk, bc);
                 const double2 rb = sR[kk * LD + bc], ra = sR[ar * LD + kk];
@@ -436,9 +439,11 @@ __global__ __launch_bounds__(512, 2) void stage_mfma64(StageArgs A) {
         cr[t] = dbl4{0.0, 0.0, 0.0, 0.0};
         ci[t] = dbl4{0.0, 0.0, 0.0, 0.0};
     }
+    // contraction over k < N only (rounded up to the MFMA's 4): the padding holds zeros
+    const int kmax = (N + 3) & ~3;
     if (vreal) {
-#pragma unroll 4
-        for (int k0 = 0; k0 < NP; k0 += 4) {
+#pragma unroll 2
+        for (int k0 = 0; k0 < kmax; k0 += 4) {
             const int kk = k0 + lk;
             const double va = Vr[ar * NP + kk];
             const double2 ra = lds_at(sR, ar, kk, N, LD);
@@ -455,7 +460,7 @@ __global__ __launch_bounds__(512, 2) void stage_mfma64(StageArgs A) {
         }
     } else {
 #pragma unroll 2
-        for (int k0 = 0; k0 < NP; k0 += 4) {
+        for (int k0 = 0; k0 < kmax; k0 += 4) {
             const int kk = k0 + lk;
             const double var = Vr[ar * NP + kk], vai = Vi[ar * NP + kk];
             const double2 ra = lds_at(sR, ar, kk, N, LD);
