@@ -69,3 +69,21 @@ def test_grid_gradient_matches_oracle_fd():
         ref = (4 * central(h0 / 2) - central(h0)) / 3
         got = float(params[name].grad[idx])
         assert abs(got - ref) <= 1e-6 * abs(ref), (name, idx, got, ref)
+
+
+def test_forward_without_gradient_keeps_no_adjoint_state():
+    """Under torch.no_grad the propagation runs the plain forward (no trajectory or
+    checkpoints): same signal as the gradient run's forward, and no backward."""
+    ctx = E.Context(0)
+    spec = SP.grid_spec(ctx, n_t1=2, t1_step=10.0, t2=20.0, n_t3=5, t3_step=5.0, L=2, checkpoint=16, taup=8.0,
+                        E0_cm=200.0)
+    params = SP.trimer_params()
+    S_grad = SP.grid_signal(params, spec)
+    assert spec.timing["forward_adjoint"] is True
+    with torch.no_grad():
+        S_plain = SP.grid_signal(params, spec)
+    assert spec.timing["forward_adjoint"] is False
+    assert torch.equal(S_grad.detach(), S_plain)
+    frozen = {k: v.detach() for k, v in params.items()}
+    S_frozen = SP.grid_signal(frozen, spec)                     # no input requires grad
+    assert spec.timing["forward_adjoint"] is False and not S_frozen.requires_grad
