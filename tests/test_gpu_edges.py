@@ -1,0 +1,111 @@
+"""GPU parity at the edges of the supported shapes, against the CPU oracle (<= 1e-10
+normwise relative, FP64):
+
+* N = 64, the largest system (no padding in the 64 x 64 MFMA tiles), with a random
+  Hermitian H and random dipoles;
+* L = 0: the hierarchy is rho_0 alone (no neighbour links);
+* K = 0: no bath at all (d = 0, one ADO), including N = 1;
+* instances of different lengths and N classes in one batch, each read out to its own
+  end (ragged step counts).
+"""
+import numpy as np
+import pytest
+
+import etoheom_amd as E
+import oracle as O
+from etoheom_amd import configs
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return E.Context(0)
+
+
+def oproblem(p):
+    return O.Problem(H=p.H, X=p.X, alpha=p.alpha, alpha_t=p.alpha_t, gamma=p.gamma, L=p.L,
+                     site_offset=p.site_offset, pulses=p.pulses, t_start=p.t_start, t_end=p.t_end,
+                     step_size=p.step_size, print_step=p.print_step)
+
+
+def rel(a, b):
+    b = np.asarray(b)
+    assert b.size and np.max(np.abs(b)) > 0, "reference is identically zero: the check would be vacuous"
+    return np.max(np.abs(np.asarray(a) - b)) / np.max(np.abs(b))
+
+
+def like(p, **over):
+    kw = dict(H=p.H, X=p.X, alpha=p.alpha, alpha_t=p.alpha_t, gamma=p.gamma, L=p.L, site_offset=p.site_offset,
+              pulses=p.pulses.copy(), t_start=p.t_start, t_end=p.t_end, step_size=p.step_size,
+              print_step=p.print_step)
+    kw.update(over)
+    return E.Problem(**kw)
+
+
+def strong(p, taus=(2.0, 5.0, 8.0), taup=3.0):
+    p.pulses[:, 1] = taus
+    p.pulses[:, 2] = taup
+    p.pulses[:, 0] *= 1e5
+    return p
+
+
+def check(ctx, p, driven=True):
+    P, rho = E.twoD_spectrum_solver(ctx, p, p.H, 1, 1, 1, X=p.X, want_rho=True)
+    Pr, rr = O.twod(oproblem(p), nv=(1, 1, 1), want_rho=True)
+    if driven and len(rr) > 1:
+        assert np.max(np.abs(rr[1:])) > 1e-12 * np.max(np.abs(rr[0]))
+    assert rel(P, Pr) < TOL
+    assert rel(rho, rr) < TOL
+
+
+def test_n64_random_hermitian_system(ctx):
+    base = configs.chain(L=1, nsteps=24)
+    rng = np.random.default_rng(64)
+    N = 64
+    A = rng.normal(size=(N, N)) * 0.02
+    H = np.diag(rng.uniform(2.2, 2.45, N)) + (A + A.T) / 2
+    H[0, :] = 0.0
+    H[:, 0] = 0.0
+    X = np.zeros((4, N, N), complex)
+    for q in range(4):
+        v = rng.normal(size=N - 1)
+        X[q, 0, 1:] = v
+        X[q, 1:, 0] = v
+    p = strong(like(base, H=H.astype(complex), X=X, alpha=base.alpha[:3], alpha_t=base.alpha_t[:3],
+                    gamma=base.gamma[:3]))
+    assert p.N == 64
+    check(ctx, p)
+
+
+def test_depth_zero_hierarchy(ctx):
+    p = strong(configs.dimer(L=0, nsteps=60))
+    check(ctx, p, driven=False)
+
+
+def test_no_bath(ctx):
+    p = configs.dimer(L=2, nsteps=60)
+    empty = np.zeros((0, 0), complex)
+    check(ctx, strong(like(p, alpha=empty, alpha_t=empty, gamma=empty, L=2)), driven=False)
+    one = strong(like(p, H=np.array([[0.3 + 0.0j]]), X=np.ones((4, 1, 1), complex), alpha=empty, alpha_t=empty,
+                      gamma=empty, L=1))
+    P, rho = E.twoD_spectrum_solver(ctx, one, one.H, 1, 1, 1, X=one.X, want_rho=True)
+    Pr, rr = O.twod(oproblem(one), nv=(1, 1, 1), want_rho=True)
+    assert len(P) == len(Pr) > 0
+    assert rel(P, Pr) < TOL
+    assert rel(rho, rr) < TOL
+
+
+def test_ragged_batch_each_instance_to_its_own_end(ctx):
+    items = [strong(configs.dimer(L=2, nsteps=37)), strong(configs.aggregate(3, 2, nsteps=301)),
+             strong(configs.dimer(L=3, nsteps=1)), strong(configs.aggregate(2, 2, nsteps=129))]
+    b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)} for p in items])
+    b.run()
+    b.sync()
+    for i, p in enumerate(items):
+        Pr, rr = O.twod(oproblem(p), nv=(1, 1, 1), want_rho=True)
+        assert len(b.output(i)) == len(Pr), i
+        if len(Pr):
+            assert rel(b.output(i), Pr) < TOL, i
+        assert rel(b.rho(i), rr) < TOL, i
