@@ -1302,13 +1302,13 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
             add(cls, 256, false);
             add(cls, 1024, false);
             add(0, 0, false);
-        } else if (cls == 17) {   // base: two-phase gathers with column copies where the instances allow
+        } else if (cls == 17) {   // base: two-phase gathers where the instances allow
+            if (use_mfma && tp_ok) add(-34, 0, false);
             if (use_mfma && tp_ok) add(-34, 0, true);
             if (use_mfma) add(-32, 0, true);
             add(0, 0, true);
             if (use_mfma) add(-32, 0, false);
             if (pipe_ok) add(-33, 0, false);
-            if (use_mfma && tp_ok) add(-34, 0, false);
         } else {   // 32 < N <= 64: stage_mfma64 gathers columns directly (measured faster at N = 56)
             if (use_mfma) add(-65, 0, false);
             add(0, 0, true);
