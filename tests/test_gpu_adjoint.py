@@ -173,3 +173,27 @@ def test_adjoint_chain_n56(ctx):
         ref = fd(p, w, name, idx, part)
         got = adj_value(g, name, idx, part)
         assert abs(got - ref) / max(abs(ref), 1e-12) < 1e-6, (name, idx, got, ref)
+
+
+def test_adjoint_replay_mode_large_n(ctx, monkeypatch):
+    """Checkpoints + replay at N = 29: the replayed segments run the per-launch stage
+    kernels with the prologue, the forward the fused graph steps."""
+    monkeypatch.setenv("ETOHEOM_ADJ_REPLAY", "1")
+    p = configs.heptamer(L=1, nsteps=40)
+    p.pulses[:, 1] = [2.0, 6.0, 10.0]
+    p.pulses[:, 2] = 3.0
+    p.pulses[:, 0] *= 3e5
+    P0, _ = O.twod(oproblem(p), nv=(1, 1, 1))
+    rng = np.random.default_rng(5)
+    w = rng.normal(size=len(P0)) + 1j * rng.normal(size=len(P0))
+    b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)}], adjoint=7)
+    b.run()
+    b.sync()
+    assert np.max(np.abs(b.output(0) - P0)) / np.max(np.abs(P0)) < 1e-10
+    b.backward([w])
+    g = b.grad(0)
+    for name, idx, part in [("H", (3, 3), "re"), ("X", (1, 0, 5), "re"), ("gamma", (1, 0), "re"),
+                            ("alpha_t", (2, 1), "im")]:
+        ref = fd(p, w, name, idx, part)
+        got = adj_value(g, name, idx, part)
+        assert abs(got - ref) / max(abs(ref), 1e-12) < 1e-6, (name, idx, got, ref)
