@@ -1308,7 +1308,7 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
             if (use_mfma) add(-32, 0, true);
             add(0, 0, true);
             if (use_mfma) add(-32, 0, false);
-            if (pipe_ok) add(-33, 0, false);
+            if (use_mfma && pipe_ok) add(-33, 0, false);
         } else {   // 32 < N <= 64: stage_mfma64 gathers columns directly (measured faster at N = 56)
             if (use_mfma) add(-65, 0, false);
             add(0, 0, true);
@@ -1533,8 +1533,7 @@ int batch_autotune(Batch* b, int steps) {
     HIPCHK(hipEventCreate(&e1));
     int rc = EH_OK;
     for (Group& g : im->groups) {
-        g.cand_ms.assign(g.cand.size(), 0.0);
-        if (g.cand.size() < 2) continue;
+        g.cand_ms.assign(g.cand.size(), 0.0);   // (a single candidate is timed too: the report shows it)
         for (size_t c = 0; c < g.cand.size() && rc == EH_OK; c++) {
             for (int st = 0; st < 4 && rc == EH_OK; st++)   // warm-up step
                 rc = launch_cfg(g, g.cand[c], stage_io(sa, im, 0, st), im->stream);

@@ -2,6 +2,8 @@
 candidates and the autotuner, each against the CPU oracle (<= 1e-10 normwise
 relative, FP64) on one batch that holds five N classes (N = 4, 7, 11, 29, 37) and
 an instance with three modes per bath."""
+import os
+
 import numpy as np
 import pytest
 
@@ -83,7 +85,8 @@ def test_every_launch_candidate_matches_oracle(ctx, refs):
     b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)} for p in items])
     cands = b.candidates()
     kinds = {k for _, _, k, _ in cands}
-    assert {"stage_small", "stage_generic", "stage_mfma32", "stage_mfma64"} <= kinds
+    mfma = {"stage_mfma32", "stage_mfma64"} if os.environ.get("ETOHEOM_NO_MFMA") != "1" else set()
+    assert {"stage_small", "stage_generic"} | mfma <= kinds
     assert {t for _, _, k, t in cands if k == "stage_small"} == {256, 512, 1024}
     for g, c, kern, tile in cands:
         print(f"group {g} candidate {c}: {kern} tile={tile}", flush=True)
