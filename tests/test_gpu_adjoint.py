@@ -197,3 +197,29 @@ def test_adjoint_replay_mode_large_n(ctx, monkeypatch):
         ref = fd(p, w, name, idx, part)
         got = adj_value(g, name, idx, part)
         assert abs(got - ref) / max(abs(ref), 1e-12) < 1e-6, (name, idx, got, ref)
+
+
+def test_adjoint_three_modes_per_bath(ctx):
+    """K_m = 3: the modes beyond the two-phase gathers go through gather_rest in the
+    adjoint's small-N kernel as well."""
+    p = configs.aggregate(2, 2, nsteps=100, taus=(10.0, 20.0, 30.0))
+    p.alpha = np.concatenate([p.alpha, 0.5 * p.alpha[:, :1]], axis=1) * 8.0
+    p.alpha_t = np.concatenate([p.alpha_t, 0.5 * p.alpha_t[:, :1]], axis=1) * 8.0
+    p.gamma = np.concatenate([p.gamma, 2.0 * p.gamma[:, :1]], axis=1)
+    p.pulses[:, 2] = 8.0
+    p.pulses[:, 0] *= 3e5
+    assert p.alpha.shape == (2, 3)
+    P0, _ = O.twod(oproblem(p), nv=(1, 1, 1))
+    rng = np.random.default_rng(6)
+    w = rng.normal(size=len(P0)) + 1j * rng.normal(size=len(P0))
+    b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)}], adjoint=16)
+    b.run()
+    b.sync()
+    assert np.max(np.abs(b.output(0) - P0)) / np.max(np.abs(P0)) < 1e-10
+    b.backward([w])
+    g = b.grad(0)
+    for name, idx, part in [("H", (1, 1), "re"), ("gamma", (0, 2), "re"), ("alpha", (1, 2), "re"),
+                            ("alpha_t", (0, 2), "im"), ("X", (0, 0, 2), "re")]:
+        ref = fd(p, w, name, idx, part)
+        got = adj_value(g, name, idx, part)
+        assert abs(got - ref) / max(abs(ref), 1e-12) < 1e-6, (name, idx, got, ref)
