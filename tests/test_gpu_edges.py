@@ -6,7 +6,9 @@ normwise relative, FP64):
 * L = 0: the hierarchy is rho_0 alone (no neighbour links);
 * K = 0: no bath at all (d = 0, one ADO), including N = 1;
 * instances of different lengths and N classes in one batch, each read out to its own
-  end (ragged step counts).
+  end (ragged step counts);
+* population-dynamics instances with different site lists beside a 2DES instance in one
+  batch (more output sites than N is an error, as in the oracle).
 """
 import numpy as np
 import pytest
@@ -109,3 +111,31 @@ def test_ragged_batch_each_instance_to_its_own_end(ctx):
         if len(Pr):
             assert rel(b.output(i), Pr) < TOL, i
         assert rel(b.rho(i), rr) < TOL, i
+
+
+def test_mixed_modes_and_site_lists_in_one_batch(ctx):
+    """Population dynamics with different site lists next to a 2DES propagation in one
+    batch (the dynamics instances: site offset 0, V = H, readouts Re rho_0[s-1][s-1])."""
+    k = E.Param.from_key("tests/golden/key_0_0.key", E.MODE_DYNAMICS)
+    H = k.get("Hal")
+    base = dict(H=H, alpha=k.get("alpha"), alpha_t=k.get("alpha_t"), gamma=k.get("gamma"), L=k.L)
+    d1 = E.Problem(**base, site_offset=0, t_start=0, t_end=400, step_size=2, print_step=10)
+    d2 = E.Problem(**base, site_offset=0, t_start=0, t_end=300, step_size=2, print_step=6)
+    tw = strong(configs.dimer(L=2, nsteps=80))
+    items = [{"problem": d1, "H": H, "mode": E.MODE_DYNAMICS, "sites": [1, 2]},
+             {"problem": tw, "H": tw.H, "X": tw.X, "nv": (1, 1, 1)},
+             {"problem": d2, "H": H, "mode": E.MODE_DYNAMICS, "sites": [2, 1]}]
+    b = E.Batch(ctx, items)
+    b.run()
+    b.sync()
+    for i, sites, p in ((0, (1, 2), d1), (2, (2, 1), d2)):
+        q = O.Problem(H=H, alpha=p.alpha, alpha_t=p.alpha_t, gamma=p.gamma, L=p.L, site_offset=0,
+                      t_start=p.t_start, t_end=p.t_end, step_size=p.step_size, print_step=p.print_step)
+        popr, _ = O.dynamics(q, sites=sites)
+        pop = b.output(i)
+        assert pop.shape == popr.shape, i
+        assert rel(pop, popr) < TOL, i
+    Pr, _ = O.twod(oproblem(tw), nv=(1, 1, 1))
+    assert rel(b.output(1), Pr) < TOL
+    with pytest.raises(Exception):
+        E.Batch(ctx, [{"problem": d2, "H": H, "mode": E.MODE_DYNAMICS, "sites": [2, 1, 2]}])
