@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) void stage_small(StageArgs A) {
         const double2 cpl = coupling_of<NT>(I, sC + nl * cpn, ra, cb);
         dr += cpl.x;
         di += cpl.y;
-        rk_store(A, I, (size_t)(base + e), st == 0 ? r : yv[q], st == 0 ? make_double2(0.0, 0.0) : av[q], dr, di,
+        rk_store<true>(A, I, (size_t)(base + e), st == 0 ? r : yv[q], st == 0 ? make_double2(0.0, 0.0) : av[q], dr, di,
                  a0 + nl, ra, cb);
     }
     if (A.fused && a0 == 0) fused_tail(A, I, step, threadIdx.x, blockDim.x);
@@ -873,6 +873,7 @@ int launch_cfg(const Group& g, const LaunchCfg& c, const StageArgs& base, hipStr
         a.cin = nullptr;
         a.cout = nullptr;
     }
+    a.stream = c.stream ? 1 : 0;
     a.tile_inst += c.tile_begin;
     a.tile_ado0 += c.tile_begin;
     a.tile_nado += c.tile_begin;
@@ -1237,6 +1238,14 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
         g.maxnn = maxnn;
         g.maxkn = maxkn;
         g.maxd = maxd;
+        // Non-temporal state stores in the small-N kernels by default when the group's four
+        // state buffers exceed 512 MB, twice the last-level cache (measured on the 2D grid:
+        // the 1.6 GB N = 11 group's stage 3 % faster with them, the 224 MB N = 7 group's
+        // 12 % slower; the MFMA kernels: the 165 MB heptamer 19 % slower, the 355 MB chain
+        // unchanged, so they never stream); the autotuner tries the other.
+        double gbytes = 0.0;
+        for (int i : members) gbytes += 4.0 * sizeof(double2) * hinst[(size_t)i].M * hinst[(size_t)i].N * hinst[(size_t)i].N;
+        const bool stream = gbytes > 512.0 * (1 << 20);
         g.h = f0.h;
         g.h2 = f0.h2;
         for (int k = 0; k < 4; k++) g.hb[k] = f0.hb[k];
@@ -1261,11 +1270,12 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
             return r;
         };
         const size_t cpb = (size_t)maxkn * sizeof(double2);   // coupling rows/columns
-        auto add = [&](int nt, int ge, bool cols) {
+        auto add = [&](int nt, int ge, bool cols, bool stream) {
             LaunchCfg c;
             c.nt = nt;
             c.ge = ge;
             c.cols = cols;
+            c.stream = stream;
             const int G = nt > 0 ? (maxnn >= ge ? 1 : ge / maxnn) : 1;
             const std::pair<int, int> r = tiles(G);
             c.tile_begin = r.first;
@@ -1298,25 +1308,27 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
         // column copies: measured to pay off for N > 16 (long strided columns), not below,
         // where the small-N kernel has no column-copy path
         if (cls <= 16) {
-            add(cls, 512, false);   // the base configuration: the adjoint's small kernels tile as it does
-            add(cls, 256, false);
-            add(cls, 1024, false);
-            add(0, 0, false);
+            add(cls, 512, false, stream);   // the base configuration: the adjoint's small kernels tile as it does
+            add(cls, 256, false, stream);
+            add(cls, 1024, false, stream);
+            add(0, 0, false, stream);
         } else if (cls == 17) {   // base: two-phase gathers where the instances allow
-            if (use_mfma && tp_ok) add(-34, 0, false);
-            if (use_mfma && tp_ok) add(-34, 0, true);
-            if (use_mfma) add(-32, 0, true);
-            add(0, 0, true);
-            if (use_mfma) add(-32, 0, false);
-            if (use_mfma && pipe_ok) add(-33, 0, false);
+            if (use_mfma && tp_ok) add(-34, 0, false, stream);
+            if (use_mfma && tp_ok) add(-34, 0, true, stream);
+            if (use_mfma) add(-32, 0, true, stream);
+            add(0, 0, true, stream);
+            if (use_mfma) add(-32, 0, false, stream);
+            if (use_mfma && pipe_ok) add(-33, 0, false, stream);
         } else {   // 32 < N <= 64: stage_mfma64 gathers columns directly (measured faster at N = 56)
-            if (use_mfma) add(-65, 0, false);
-            add(0, 0, true);
+            if (use_mfma) add(-65, 0, false, stream);
+            add(0, 0, true, stream);
             if (use_mfma) {
-                add(-65, 0, true);
-                add(-64, 0, true);
+                add(-65, 0, true, stream);
+                add(-64, 0, true, stream);
             }
         }
+        // the small-N base configuration with the other store hint (the autotuner's choice)
+        if (cls <= 16 && !g.cand.empty()) add(g.cand[0].nt, g.cand[0].ge, g.cand[0].cols, !stream);
         if (g.cand.empty() || (cls <= 16 && g.cand[0].nt != cls))
             return fail(EH_ESHAPE, "state tile exceeds LDS");
         for (size_t c = 0; c < g.cand.size(); c++)
@@ -1590,8 +1602,9 @@ std::string batch_tuning_report(const Batch* b) {
                                            : "stage_generic";
             char line[256];
             std::snprintf(line, sizeof(line),
-                          "group %zu cand %zu maxN^2=%d instances=%d %s tile=%d tiles=%d cols=%d ms_per_step=%.6f%s\n",
+                          "group %zu cand %zu maxN^2=%d instances=%d %s tile=%d tiles=%d cols=%d stream=%d ms_per_step=%.6f%s\n",
                           gi, c, g.maxnn, g.inst_count, kn, x.nt > 0 ? x.ge : 0, x.tile_count, x.cols ? 1 : 0,
+                          x.stream ? 1 : 0,
                           g.cand_ms.empty() ? -1.0 : g.cand_ms[c], (int)c == g.sel ? " *" : "");
             out += line;
         }

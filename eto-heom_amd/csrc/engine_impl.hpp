@@ -56,6 +56,7 @@ struct LaunchCfg {
                               // -33: stage_mfma_pipe; 0: stage_generic
     int ge;                   // stage_small: state elements per tile (256, 512 or 1024)
     bool cols;                // keep coupled-column copies (contiguous column gathers)
+    bool stream;              // non-temporal state stores (StageArgs::stream)
     int tile_begin, tile_count;
     size_t shmem;
 };

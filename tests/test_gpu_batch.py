@@ -88,6 +88,10 @@ def test_every_launch_candidate_matches_oracle(ctx, refs):
     mfma = {"stage_mfma32", "stage_mfma64"} if os.environ.get("ETOHEOM_NO_MFMA") != "1" else set()
     assert {"stage_small", "stage_generic"} | mfma <= kinds
     assert {t for _, _, k, t in cands if k == "stage_small"} == {256, 512, 1024}
+    # every small-N group also offers its base kernel with the other state-store hint
+    rep = b.tuning_report().splitlines()
+    for g in {g for g, _, k, _ in cands if k == "stage_small"}:
+        assert {l.split("stream=")[1][0] for l in rep if int(l.split()[1]) == g} == {"0", "1"}
     for g, c, kern, tile in cands:
         print(f"group {g} candidate {c}: {kern} tile={tile}", flush=True)
         b.select(g, c)
