@@ -229,14 +229,12 @@ __global__ __launch_bounds__(256) void adj_small(AdjArgs J) {
     constexpr int NN = NT * NT;
     constexpr int G = NN >= 512 ? 1 : 512 / NN;
     constexpr int EPT = (G * NN + 255) / 256;
-    constexpr int CPMAX = G * 2 * NT * NT;
     constexpr int DMAX = 64;
     __shared__ double2 sR[G * NN];
     __shared__ double2 sY[G * NN];
     __shared__ double2 sV[NN];
-    __shared__ double2 sC[CPMAX];
+    extern __shared__ double2 sC[];   // G * 2 K N coupling rows/columns (launch-time size)
     __shared__ double2 sG[G * NN];
-    __shared__ double2 sQ[G * NN];
     __shared__ double sU[3 * 2 * DMAX];
     __shared__ double sSg[2 * DMAX];
     const int tile = blockIdx.x;
@@ -342,8 +340,6 @@ __global__ __launch_bounds__(256) void adj_small(AdjArgs J) {
             gt.y += y1.y * k1.x - y1.x * k1.y - (k2.x * y2.y - k2.y * y2.x);
         }
         sG[e] = make_double2(fs * gt.x, fs * gt.y);
-        const double2 qv = cconj_mul(R[ab], Y[ab]);
-        sQ[e] = make_double2(fs * qv.x, fs * qv.y);
         double ur = -ci, ui = cr;   // +i * C'
         const double2 g = gv[q], r = R[ab];
         ur = fma(g.x, r.x, fma(-g.y, r.y, ur));
@@ -379,7 +375,8 @@ __global__ __launch_bounds__(256) void adj_small(AdjArgs J) {
         }
     }
     const int d = I.d;
-    // q_n = sum_ab sQ[n][ab] in two levels (8 partial sums per ADO), then n_p q_n per (n, p)
+    // q_n = fs sum_ab conj(kb_n[ab]) Y_n[ab] in two levels (8 partial sums per ADO, from the
+    // stage input and the forward state still in LDS), then n_p q_n per (n, p)
     constexpr int QP = NN < 8 ? NN : 8, QC = (NN + QP - 1) / QP;   // na * QP <= G * NN: fits sG
     double2* sPart = sG;   // the Gt partials are consumed above; reuse their LDS
     __syncthreads();
@@ -390,11 +387,12 @@ __global__ __launch_bounds__(256) void adj_small(AdjArgs J) {
         for (int c = 0; c < QC; ++c) {
             const int ab = part * QC + c;
             if (ab < NN) {
-                qs.x += sQ[n * NN + ab].x;
-                qs.y += sQ[n * NN + ab].y;
+                const double2 qv = cconj_mul(sR[n * NN + ab], sY[n * NN + ab]);
+                qs.x += qv.x;
+                qs.y += qv.y;
             }
         }
-        sPart[t] = qs;
+        sPart[t] = make_double2(fs * qs.x, fs * qs.y);
     }
     __syncthreads();
     for (int t = threadIdx.x; t < na * d; t += 256) {   // n_p q_n summed over the tile's ADOs in LDS
@@ -659,7 +657,7 @@ static int launch_adj(const Group& g, const AdjArgs& base, hipStream_t s) {
     const dim3 grid(g.tile_count), block(256);
     switch (g.nt) {
 #define EH_CASE(n) \
-    case n: hipLaunchKernelGGL(adj_small<n>, grid, block, 0, s, a); break;
+    case n: hipLaunchKernelGGL(adj_small<n>, grid, block, g.shmem, s, a); break;
         EH_CASE(1) EH_CASE(2) EH_CASE(3) EH_CASE(4) EH_CASE(5) EH_CASE(6) EH_CASE(7) EH_CASE(8)
         EH_CASE(9) EH_CASE(10) EH_CASE(11) EH_CASE(12) EH_CASE(13) EH_CASE(14) EH_CASE(15) EH_CASE(16)
 #undef EH_CASE
