@@ -120,33 +120,39 @@ __device__ __forceinline__ double2 cmad(double2 acc, double2 a, double2 b) {
     return acc;
 }
 
-__device__ __forceinline__ double stage_hb(const InstDev& I, int st) {
-    return st == 0 ? I.hb[0] : (st == 1 ? I.hb[1] : (st == 2 ? I.hb[2] : I.hb[3]));
-}
-
 // RK4 stage update of one element from its RHS value k (the reference's classic RK4,
-// dynamics.cpp:386-417, in 8 reads + 6 writes of the state per step instead of 9 + 7):
-//   stage 0: out = y + h/2 k                       (no ACC write)
-//   stage 1: ACC = y + (x - y)/3 + h/3 k, out = y + h/2 k, where x = y + h/2 k1 is this
-//            stage's input, passed as `a` (so y + (x - y)/3 = y + h/6 k1: ACC is rebuilt
-//            here instead of written by stage 0 and read back)
-//   stage 2: ACC += h/3 k, out = y + h k;  stage 3: out (= Y) = ACC + h/6 k
+// dynamics.cpp:386-417, in 8 reads + 5 writes of the state per step instead of 9 + 7).
+// x is the element of this stage's input, y of Y, a of ACC (each read where used):
+//   stage 0: out = x + h/2 k                      (x = y)
+//   stage 1: ACC = y + (x - y)/3 + h/3 k,  out = y + h/2 k
+//            (x = y + h/2 k1, so y + (x - y)/3 = y + h/6 k1: stage 0 writes no ACC)
+//   stage 2: out = y + h k                        (no ACC update ...)
+//   stage 3: out (= Y) = ACC + (x - y)/3 + h/6 k  (... stage 3 adds h/3 k3 from its own
+//            input x = y + h k3)
+// A kernel with no registers to spare passes two operands instead of three (x unused):
+// y (x in stage 0) and a (x in stage 1; in stage 3, with TWO = 1, ACC + (x - y)/3 folded
+// by the caller).  TWO = 2 is the 8 reads + 6 writes schedule in that form: stage 2 also
+// does ACC += h/3 k and stage 3 out = ACC + h/6 k (Y read in stages 1-2, ACC in 2-3).
+// All four stages of a step run one kernel.
 // Element (row, col) of ADO m (instance-local index) also goes to the column copy.
-template <bool ST = false>
-__device__ __forceinline__ void rk_store(const StageArgs& A, const InstDev& I, size_t gi, double2 y, double2 a,
-                                         double kr, double ki, int m, int row, int col) {
+template <bool ST = false, int TWO = 0>
+__device__ __forceinline__ void rk_store(const StageArgs& A, const InstDev& I, size_t gi, double2 x, double2 y,
+                                         double2 a, double kr, double ki, int m, int row, int col) {
     const int st = A.stage;
-    const double hb = stage_hb(I, st);
+    if (TWO) x = st == 0 ? y : a;
     double2 o;
     if (st == 0) {
+        o = make_double2(fma(I.h2, kr, x.x), fma(I.h2, ki, x.y));
+    } else if (st == 1) {
+        const double2 b = make_double2(fma(x.x - y.x, 1.0 / 3.0, y.x), fma(x.y - y.y, 1.0 / 3.0, y.y));
+        st_state<ST>(A, A.acc + gi, make_double2(fma(I.hb[1], kr, b.x), fma(I.hb[1], ki, b.y)));
         o = make_double2(fma(I.h2, kr, y.x), fma(I.h2, ki, y.y));
-    } else if (st < 3) {
-        const double hs = st == 1 ? I.h2 : I.h;
-        const double2 b = st == 1 ? make_double2(fma(a.x - y.x, 1.0 / 3.0, y.x), fma(a.y - y.y, 1.0 / 3.0, y.y)) : a;
-        st_state<ST>(A, A.acc + gi, make_double2(fma(hb, kr, b.x), fma(hb, ki, b.y)));
-        o = make_double2(fma(hs, kr, y.x), fma(hs, ki, y.y));
+    } else if (st == 2) {
+        if (TWO == 2) st_state<ST>(A, A.acc + gi, make_double2(fma(I.hb[2], kr, a.x), fma(I.hb[2], ki, a.y)));
+        o = make_double2(fma(I.h, kr, y.x), fma(I.h, ki, y.y));
     } else {
-        o = make_double2(fma(hb, kr, a.x), fma(hb, ki, a.y));
+        const double2 b = TWO ? a : make_double2(fma(x.x - y.x, 1.0 / 3.0, a.x), fma(x.y - y.y, 1.0 / 3.0, a.y));
+        o = make_double2(fma(I.hb[3], kr, b.x), fma(I.hb[3], ki, b.y));
     }
     st_state<ST>(A, A.out + gi, o);
     const int jc = col - I.so;
@@ -165,12 +171,12 @@ __device__ __forceinline__ const double2* nb_elem(const StageArgs& A, const Inst
     return (A.rin + I.state_off) + (unsigned)(mu * N * N + (rc == 0 ? s * N + pos : pos * N + s));
 }
 
-// y and the `a` operand of rk_store for element gi as the stage needs them (stage 0
-// takes y = rin value; stage 1 takes a = rin value)
+// the y and a operands of rk_store for element gi as the stage needs them (Y from
+// stage 1 on, ACC in stage 3; otherwise the input value)
 __device__ __forceinline__ void rk_load(const StageArgs& A, size_t gi, double2 rin_val, double2& y, double2& a) {
     const int st = A.stage;
-    y = st == 0 ? rin_val : (st < 3 ? A.y[gi] : make_double2(0.0, 0.0));
-    a = st >= 2 ? A.acc[gi] : rin_val;
+    y = st == 0 ? rin_val : A.y[gi];
+    a = st == 3 ? A.acc[gi] : rin_val;
 }
 
 // Coupling gathers of one ADO into LDS.  For bath j (site s = j + so), the
