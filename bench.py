@@ -259,10 +259,10 @@ def main():
     stage_ms = float(np.mean(prof[1:5]))
     elems = sum(p.num_ado() * p.N * p.N for p in items)
     # the algorithmic model: 256 N^2 B per ADO-update (9 reads + 7 writes of the N x N
-    # block per RK4 step) over 4 stage launches; the kernels' own schedule needs 8 + 5
-    # (208 N^2 B), reported beside it
+    # block per RK4 step) over 4 stage launches; the kernels' own schedule needs 7 + 5
+    # (192 N^2 B), reported beside it
     bytes_per_launch = 64.0 * elems
-    schedule_bytes = 52.0 * elems
+    schedule_bytes = 48.0 * elems
     achieved = bytes_per_launch / (stage_ms * 1e-3) / 1e9
 
     per_prop = per_propagation(ctx, items, a.per_propagation) if a.per_propagation else None

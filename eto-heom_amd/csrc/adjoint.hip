@@ -74,8 +74,9 @@ __device__ __forceinline__ double2 cconj_mul(double2 a, double2 b) {   // conj(a
 // the stage's input x, lam and ACC (each read where used).  Two stages write no
 // accumulator: stage 0's output is x1 = h/3 lam + h^2/6 u0, so stage 1 rebuilds
 // lam + h/6 u0 = 2/3 lam + x1/h from its own input; stage 2's output is
-// x3 = h/2 u2 + h/6 lam, so stage 3 adds u2 = 2/h x3 - lam/3 from its own input
-// (5 state writes per step instead of 6).
+// x3 = h/2 u2 + h/6 lam, so stage 3 adds u2 = 2/h x3 - lam/3 from its own input.  ACC
+// holds the classic accumulator less lam/3, so stage 3 reads ACC but no lam (5 state
+// writes per step instead of 6).
 __device__ __forceinline__ void rk_store_adj(const StageArgs& A, const InstDev& I, size_t gi, double2 x, double2 lam,
                                              double2 acc, double ur, double ui) {
     const int st = A.stage;
@@ -87,13 +88,13 @@ __device__ __forceinline__ void rk_store_adj(const StageArgs& A, const InstDev& 
     } else if (st < 3) {
         const double b = st == 1 ? h / 3.0 : h / 6.0, c = h / 2.0;
         if (st == 1) {
-            const double t = 2.0 / 3.0;
+            const double t = 1.0 / 3.0;
             A.acc[gi] = make_double2(fma(t, lam.x, fma(x.x, ih, ur)), fma(t, lam.y, fma(x.y, ih, ui)));
         }
         A.out[gi] = make_double2(fma(c, ur, b * lam.x), fma(c, ui, b * lam.y));
     } else {
-        const double t = 2.0 * ih, s = -1.0 / 3.0;
-        A.out[gi] = make_double2(fma(s, lam.x, fma(t, x.x, acc.x + ur)), fma(s, lam.y, fma(t, x.y, acc.y + ui)));
+        const double t = 2.0 * ih;
+        A.out[gi] = make_double2(fma(t, x.x, acc.x + ur), fma(t, x.y, acc.y + ui));
     }
 }
 
@@ -267,7 +268,7 @@ __global__ __launch_bounds__(256) void adj_small(AdjArgs J) {
     if (has_cpl) bslot = bath_slot_load(J, I, a0, na, threadIdx.x);
     const int dq = na * I.d;
     const int np_pre = dq > 0 ? J.ado[I.lk_off + (long long)a0 * I.d + (threadIdx.x < dq ? threadIdx.x : 0)] : 0;
-    const double2* lp = st >= 1 ? A.y : A.rin;
+    const double2* lp = (st == 1 || st == 2) ? A.y : A.rin;
     const double2* ap = st == 3 ? A.acc : A.rin;
     double2 rv[EPT], lv[EPT], av[EPT], yf[EPT], gv[EPT];
 #pragma unroll
@@ -491,7 +492,7 @@ __global__ __launch_bounds__(256) void adj_generic(AdjArgs J) {
         ur = fma(g.x, r.x, fma(-g.y, r.y, ur));
         ui = fma(g.x, r.y, fma(g.y, r.x, ui));
         const double2 cpl = coupling_of(I, sC, ra, cb);
-        const double2 lam = st == 0 ? r : A.y[base + ab];
+        const double2 lam = (st == 1 || st == 2) ? A.y[base + ab] : r;
         const double2 av = st == 3 ? A.acc[base + ab] : r;
         rk_store_adj(A, I, (size_t)(base + ab), r, lam, av, ur + cpl.x, ui + cpl.y);
     }

@@ -121,21 +121,20 @@ __device__ __forceinline__ double2 cmad(double2 acc, double2 a, double2 b) {
 }
 
 // RK4 stage update of one element from its RHS value k (the reference's classic RK4,
-// dynamics.cpp:386-417, in 8 reads + 5 writes of the state per step instead of 9 + 7).
+// dynamics.cpp:386-417, in 7 reads + 5 writes of the state per step instead of 9 + 7).
 // x is the element of this stage's input, y of Y, a of ACC (each read where used):
-//   stage 0: out = x + h/2 k                      (x = y)
-//   stage 1: ACC = y + (x - y)/3 + h/3 k,  out = y + h/2 k
-//            (x = y + h/2 k1, so y + (x - y)/3 = y + h/6 k1: stage 0 writes no ACC)
-//   stage 2: out = y + h k                        (no ACC update ...)
-//   stage 3: out (= Y) = ACC + (x - y)/3 + h/6 k  (... stage 3 adds h/3 k3 from its own
-//            input x = y + h k3)
-// A kernel with no registers to spare passes two operands instead of three (x unused):
-// y (x in stage 0) and a (x in stage 1; in stage 3, with TWO = 1, ACC + (x - y)/3 folded
-// by the caller).  TWO = 2 is the 8 reads + 6 writes schedule in that form: stage 2 also
-// does ACC += h/3 k and stage 3 out = ACC + h/6 k (Y read in stages 1-2, ACC in 2-3).
-// All four stages of a step run one kernel.
+//   stage 0: out = x + h/2 k                     (x = y)
+//   stage 1: ACC = (x + y)/3 + h/3 k,  out = y + h/2 k
+//   stage 2: out = y + h k
+//   stage 3: out (= Y) = ACC + x/3 + h/6 k
+// ACC holds the classic accumulator y + h/6 k1 + h/3 k2 less y/3: stage 1 rebuilds the
+// h/6 k1 term from its own input x = y + h/2 k1 (stage 0 writes no ACC), and stage 3's
+// x/3 = y/3 + h/3 k3 restores y/3 and adds the h/3 k3 term from its own input x = y + h k3
+// (stage 2 writes no ACC, stage 3 reads no Y).
+// A kernel with no registers to spare passes two operands (TWO; x unused): y (x in stage
+// 0) and a (x in stage 1, ACC + x/3 folded by the caller in stage 3).
 // Element (row, col) of ADO m (instance-local index) also goes to the column copy.
-template <bool ST = false, int TWO = 0>
+template <bool ST = false, bool TWO = false>
 __device__ __forceinline__ void rk_store(const StageArgs& A, const InstDev& I, size_t gi, double2 x, double2 y,
                                          double2 a, double kr, double ki, int m, int row, int col) {
     const int st = A.stage;
@@ -144,14 +143,13 @@ __device__ __forceinline__ void rk_store(const StageArgs& A, const InstDev& I, s
     if (st == 0) {
         o = make_double2(fma(I.h2, kr, x.x), fma(I.h2, ki, x.y));
     } else if (st == 1) {
-        const double2 b = make_double2(fma(x.x - y.x, 1.0 / 3.0, y.x), fma(x.y - y.y, 1.0 / 3.0, y.y));
+        const double2 b = make_double2((x.x + y.x) * (1.0 / 3.0), (x.y + y.y) * (1.0 / 3.0));
         st_state<ST>(A, A.acc + gi, make_double2(fma(I.hb[1], kr, b.x), fma(I.hb[1], ki, b.y)));
         o = make_double2(fma(I.h2, kr, y.x), fma(I.h2, ki, y.y));
     } else if (st == 2) {
-        if (TWO == 2) st_state<ST>(A, A.acc + gi, make_double2(fma(I.hb[2], kr, a.x), fma(I.hb[2], ki, a.y)));
         o = make_double2(fma(I.h, kr, y.x), fma(I.h, ki, y.y));
     } else {
-        const double2 b = TWO ? a : make_double2(fma(x.x - y.x, 1.0 / 3.0, a.x), fma(x.y - y.y, 1.0 / 3.0, a.y));
+        const double2 b = TWO ? a : make_double2(fma(x.x, 1.0 / 3.0, a.x), fma(x.y, 1.0 / 3.0, a.y));
         o = make_double2(fma(I.hb[3], kr, b.x), fma(I.hb[3], ki, b.y));
     }
     st_state<ST>(A, A.out + gi, o);
@@ -171,11 +169,11 @@ __device__ __forceinline__ const double2* nb_elem(const StageArgs& A, const Inst
     return (A.rin + I.state_off) + (unsigned)(mu * N * N + (rc == 0 ? s * N + pos : pos * N + s));
 }
 
-// the y and a operands of rk_store for element gi as the stage needs them (Y from
-// stage 1 on, ACC in stage 3; otherwise the input value)
+// the y and a operands of rk_store for element gi as the stage needs them (Y in stages
+// 1-2, ACC in stage 3; otherwise the input value)
 __device__ __forceinline__ void rk_load(const StageArgs& A, size_t gi, double2 rin_val, double2& y, double2& a) {
     const int st = A.stage;
-    y = st == 0 ? rin_val : A.y[gi];
+    y = (st == 1 || st == 2) ? A.y[gi] : rin_val;
     a = st == 3 ? A.acc[gi] : rin_val;
 }
 

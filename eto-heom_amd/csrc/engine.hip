@@ -10,10 +10,11 @@
 //      stage 0: k1 = f(Y)   S1 = Y + h/2 k1
 //      stage 1: k2 = f(S1)  ACC = Y + (S1 - Y)/3 + h/3 k2   S2 = Y + h/2 k2
 //      stage 2: k3 = f(S2)  S1 = Y + h k3
-//      stage 3: k4 = f(S1)  Y = ACC + (S1 - Y)/3 + h/6 k4
+//      stage 3: k4 = f(S1)  Y = ACC - Y/3 + S1/3 + h/6 k4
 //    which is the reference's RK4 (dynamics.cpp:385-417) with V frozen over the
 //    step (the h/6 k1 and h/3 k3 terms are rebuilt from the next stage's input
-//    instead of written to ACC: 8 reads + 5 writes of each ADO per step).  In the forward graph stage 0
+//    instead of written to ACC, which holds ACC - Y/3 so that stage 3 reads no Y:
+//    7 reads + 5 writes of each ADO per step).  In the forward graph stage 0
 //    also reads out Tr(rho_0 R) before the step and stage 3 builds V of the next
 //    step (double-buffered by step parity); elsewhere a prologue kernel does both.
 //  * f(rho)_n[a][b] = -i (V rho_n - rho_n V)[a][b] + damp_n rho_n[a][b]
@@ -75,7 +76,7 @@ __global__ __launch_bounds__(256) void stage_small(StageArgs A) {
     const bool vreal = A.vreal[I.step_off + step] != 0;
     // y and acc where the stage reads them, else the (just loaded, cached) input
     const int st = A.stage;
-    const double2* yp = st >= 1 ? A.y : A.rin;
+    const double2* yp = (st == 1 || st == 2) ? A.y : A.rin;
     const double2* ap = st == 3 ? A.acc : A.rin;
     double2 rv[EPT], yv[EPT], av[EPT], gv[EPT];
 #pragma unroll
@@ -251,7 +252,7 @@ __global__ __launch_bounds__(256, NP == 32 ? (TP ? 3 : 4) : 2) void stage_mfma(S
     // y / acc of this lane's output elements
     double2 yv[TPW][4], av[TPW][4];
     if (TP) {   // every load unconditional: clamped address, selected result
-        const double2* yp = A.stage >= 1 ? A.y : A.rin;
+        const double2* yp = (A.stage == 1 || A.stage == 2) ? A.y : A.rin;
         const double2* ap = A.stage == 3 ? A.acc : A.rin;
 #pragma unroll
         for (int q = 0; q < SPT; ++q) {
@@ -299,7 +300,7 @@ __global__ __launch_bounds__(256, NP == 32 ? (TP ? 3 : 4) : 2) void stage_mfma(S
                 av[t][r] = make_double2(0.0, 0.0);
                 if (row < N && col < N && A.stage != 0) {
                     const size_t gi = (size_t)(base + row * N + col);
-                    yv[t][r] = A.y[gi];
+                    if (A.stage < 3) yv[t][r] = A.y[gi];
                     if (A.stage == 3) av[t][r] = A.acc[gi];
                 }
             }
@@ -479,7 +480,7 @@ __global__ __launch_bounds__(512, 2) void stage_mfma64(StageArgs A) {
         }
     }
     // y and acc of the lane's 8 output elements (unconditional, clamped), then the update
-    const double2* yp = st >= 1 ? A.y : A.rin;
+    const double2* yp = (st == 1 || st == 2) ? A.y : A.rin;
     const double2* ap = st == 3 ? A.acc : A.rin;
     double2 yv[2][4], av[2][4];
 #pragma unroll
@@ -506,14 +507,13 @@ __global__ __launch_bounds__(512, 2) void stage_mfma64(StageArgs A) {
                 dr = fma(g.x, x.x, fma(-g.y, x.y, dr));
                 di = fma(g.x, x.y, fma(g.y, x.x, di));
                 const double2 cpl = coupling_of(I, sC, row, col);
-                // two operands: y (x in stage 0) and a (x in stage 1, ACC + (x - y)/3 in stage 3)
-                const double2 yc = yv[t][r];
-                const double2 y0 = st == 0 ? x : yc;
-                const double2 xs = st == 3 ? make_double2((x.x - yc.x) * (1.0 / 3.0), (x.y - yc.y) * (1.0 / 3.0))
-                                           : make_double2(0.0, 0.0);
-                const double2 a1 = st == 1 ? x : make_double2(av[t][r].x + xs.x, av[t][r].y + xs.y);
-                rk_store<false, 1>(A, I, (size_t)(base + row * N + col), x, y0, a1, dr + cpl.x, di + cpl.y, a0, row,
-                                   col);
+                // two operands: y (x in stage 0) and a (x in stage 1, ACC + x/3 in stage 3)
+                const double2 y0 = st == 0 ? x : yv[t][r];
+                const double2 a1 = st == 1 ? x
+                                 : (st == 3 ? make_double2(fma(x.x, 1.0 / 3.0, av[t][r].x), fma(x.y, 1.0 / 3.0, av[t][r].y))
+                                            : av[t][r]);
+                rk_store<false, true>(A, I, (size_t)(base + row * N + col), x, y0, a1, dr + cpl.x, di + cpl.y, a0, row,
+                                      col);
             }
         }
     }
@@ -626,7 +626,7 @@ __device__ __forceinline__ void load_dat(const StageArgs& A, const Tile& T, cons
         D.a[r] = make_double2(0.0, 0.0);
         if (row < N && col < N && A.stage != 0) {
             const size_t gi = (size_t)(base + row * N + col);
-            D.y[r] = A.y[gi];
+            if (A.stage < 3) D.y[r] = A.y[gi];
             if (A.stage == 3) D.a[r] = A.acc[gi];
         }
     }
@@ -807,19 +807,18 @@ __global__ __launch_bounds__(256, 2) void stage_mfma_pipe(StageArgs A, int ntile
                         ki += v.y;
                     }
                     const size_t gi = (size_t)(base + row * N + col);
-                    // the RK4 schedule of rk_store (8 reads + 5 writes per step)
+                    // the RK4 schedule of rk_store (7 reads + 5 writes per step)
                     const double2 yc = ycur[r];
                     if (st == 0) {
                         A.out[gi] = make_double2(fma(h2, kr, x.x), fma(h2, ki, x.y));
                     } else if (st == 1) {
-                        const double2 b = make_double2(fma(x.x - yc.x, 1.0 / 3.0, yc.x), fma(x.y - yc.y, 1.0 / 3.0, yc.y));
+                        const double2 b = make_double2((x.x + yc.x) * (1.0 / 3.0), (x.y + yc.y) * (1.0 / 3.0));
                         A.acc[gi] = make_double2(fma(hb, kr, b.x), fma(hb, ki, b.y));
                         A.out[gi] = make_double2(fma(h2, kr, yc.x), fma(h2, ki, yc.y));
                     } else if (st == 2) {
                         A.out[gi] = make_double2(fma(h, kr, yc.x), fma(h, ki, yc.y));
                     } else {
-                        const double2 b = make_double2(fma(x.x - yc.x, 1.0 / 3.0, acur[r].x),
-                                                       fma(x.y - yc.y, 1.0 / 3.0, acur[r].y));
+                        const double2 b = make_double2(fma(x.x, 1.0 / 3.0, acur[r].x), fma(x.y, 1.0 / 3.0, acur[r].y));
                         A.out[gi] = make_double2(fma(hb, kr, b.x), fma(hb, ki, b.y));
                     }
                 }
