@@ -404,7 +404,12 @@ class Batch:
         self.n_inst, self.total_ados, self.total_elems, self.max_steps, self.chunk = (int(v) for v in d)
         self.ado_updates = int(lib().eh_batch_ado_updates(self._h))
         if adjoint:
-            _check(lib().eh_batch_enable_adjoint(self._h, int(adjoint)))
+            self.enable_adjoint(adjoint)
+
+    def enable_adjoint(self, checkpoint: int):
+        """Enable gradients with a checkpoint every `checkpoint` steps (after any
+        select / autotune: the backward uses the launch candidates chosen then)."""
+        _check(lib().eh_batch_enable_adjoint(self._h, int(checkpoint)))
 
     def backward(self, dP):
         """dP: list of complex arrays (dL/dRe P + i dL/dIm P), one per instance."""

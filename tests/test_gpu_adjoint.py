@@ -104,6 +104,32 @@ def test_adjoint_matches_oracle_fd_dimer(ctx, C, mode, monkeypatch):
         assert abs(got - ref) / scale < 1e-6, (name, idx, part, got, ref)
 
 
+def test_adjoint_with_streamed_store_candidate(ctx):
+    """The small-N candidate with non-temporal state stores changes cache hints only: the
+    forward readouts are bitwise those of the default candidate, and the gradients agree
+    to rounding (their reductions add in LDS with atomics, in no fixed order)."""
+    p = small_dimer()
+    P0, _ = O.twod(oproblem(p), nv=(1, 1, 1))
+    rng = np.random.default_rng(5)
+    w = rng.normal(size=len(P0)) + 1j * rng.normal(size=len(P0))
+    res = []
+    for streamed in ("stream=0", "stream=1"):
+        b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)}])
+        line = [l for l in b.tuning_report().splitlines()
+                if "stage_small" in l and "tile=512" in l and streamed in l][0]
+        b.select(int(line.split()[1]), int(line.split()[3]))
+        b.enable_adjoint(16)
+        b.run()
+        b.sync()
+        b.backward([w])
+        res.append((b.output(0), b.grad(0)))
+    assert np.max(np.abs(res[0][0] - P0)) / np.max(np.abs(P0)) < 1e-10
+    assert np.array_equal(res[0][0], res[1][0])
+    for name in res[0][1]:
+        a, b2 = res[0][1][name], res[1][1][name]
+        assert np.max(np.abs(a - b2)) <= 1e-12 * np.max(np.abs(a)), name
+
+
 def test_adjoint_batch_of_two_sizes(ctx):
     """Per-instance offsets: a dimer and a trimer in one batch, gradients of each vs FD."""
     p1 = small_dimer()
