@@ -122,7 +122,8 @@ __device__ __forceinline__ double2 cmad(double2 acc, double2 a, double2 b) {
 
 // RK4 stage update of one element from its RHS value k (the reference's classic RK4,
 // dynamics.cpp:386-417, in 7 reads + 5 writes of the state per step instead of 9 + 7).
-// x is the element of this stage's input, y of Y, a of ACC (each read where used):
+// x is the element of this stage's input, y of Y, a of ACC (each read where used; no
+// stage uses both, so a kernel may load one operand and pass it as both):
 //   stage 0: out = x + h/2 k                     (x = y)
 //   stage 1: ACC = (x + y)/3 + h/3 k,  out = y + h/2 k
 //   stage 2: out = y + h k
@@ -169,12 +170,11 @@ __device__ __forceinline__ const double2* nb_elem(const StageArgs& A, const Inst
     return (A.rin + I.state_off) + (unsigned)(mu * N * N + (rc == 0 ? s * N + pos : pos * N + s));
 }
 
-// the y and a operands of rk_store for element gi as the stage needs them (Y in stages
-// 1-2, ACC in stage 3; otherwise the input value)
-__device__ __forceinline__ void rk_load(const StageArgs& A, size_t gi, double2 rin_val, double2& y, double2& a) {
+// the one state operand of rk_store besides the input for element gi (Y in stages 1-2,
+// ACC in stage 3, the input value in stage 0): pass it as both y and a
+__device__ __forceinline__ double2 rk_operand(const StageArgs& A, size_t gi, double2 rin_val) {
     const int st = A.stage;
-    y = (st == 1 || st == 2) ? A.y[gi] : rin_val;
-    a = st == 3 ? A.acc[gi] : rin_val;
+    return st == 3 ? A.acc[gi] : ((st == 1 || st == 2) ? A.y[gi] : rin_val);
 }
 
 // Coupling gathers of one ADO into LDS.  For bath j (site s = j + so), the

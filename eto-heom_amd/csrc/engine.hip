@@ -74,19 +74,18 @@ __global__ __launch_bounds__(256) void stage_small(StageArgs A) {
     NbIdx<2, 2> nbx;
     if (has_cpl) gather_idx<2, 2, NT>(A, I, a0, na, threadIdx.x, 256, nbx);
     const bool vreal = A.vreal[I.step_off + step] != 0;
-    // y and acc where the stage reads them, else the (just loaded, cached) input
+    // the one state operand the stage's update reads besides its input (Y in stages 1-2,
+    // ACC in stage 3; stage 0 re-reads the just loaded, cached input)
     const int st = A.stage;
-    const double2* yp = (st == 1 || st == 2) ? A.y : A.rin;
-    const double2* ap = st == 3 ? A.acc : A.rin;
-    double2 rv[EPT], yv[EPT], av[EPT], gv[EPT];
+    const double2* wp = st == 3 ? A.acc : ((st == 1 || st == 2) ? A.y : A.rin);
+    double2 rv[EPT], wv[EPT], gv[EPT];
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
         const int e = threadIdx.x + q * 256;
         const int ec = e < nel ? e : nel - 1;
         const size_t gi = (size_t)(base + ec);
         rv[q] = A.rin[gi];
-        yv[q] = yp[gi];
-        av[q] = ap[gi];
+        wv[q] = wp[gi];
         gv[q] = A.damp[I.ado_off + a0 + ec / NN];
     }
     // V into registers (written to LDS with the tile: a store right after its load
@@ -147,7 +146,7 @@ __global__ __launch_bounds__(256) void stage_small(StageArgs A) {
         const double2 cpl = coupling_of<NT>(I, sC + nl * cpn, ra, cb);
         dr += cpl.x;
         di += cpl.y;
-        rk_store<true>(A, I, (size_t)(base + e), r, yv[q], av[q], dr, di,
+        rk_store<true>(A, I, (size_t)(base + e), r, wv[q], wv[q], dr, di,
                  a0 + nl, ra, cb);
     }
     if (A.fused && a0 == 0) fused_tail(A, I, step, threadIdx.x, blockDim.x);
@@ -198,9 +197,8 @@ __global__ __launch_bounds__(256) void stage_generic(StageArgs A) {
         dr = fma(g.x, r.x, fma(-g.y, r.y, dr));
         di = fma(g.x, r.y, fma(g.y, r.x, di));
         const double2 cpl = coupling_of(I, sC, ra, cb);
-        double2 yv, av;
-        rk_load(A, (size_t)(base + ab), r, yv, av);
-        rk_store(A, I, (size_t)(base + ab), r, yv, av, dr + cpl.x, di + cpl.y, a0, ra, cb);
+        const double2 wv = rk_operand(A, (size_t)(base + ab), r);
+        rk_store(A, I, (size_t)(base + ab), r, wv, wv, dr + cpl.x, di + cpl.y, a0, ra, cb);
     }
     if (A.fused && a0 == 0) fused_tail(A, I, step, threadIdx.x, blockDim.x);
 }
@@ -250,6 +248,8 @@ __global__ __launch_bounds__(256, NP == 32 ? (TP ? 3 : 4) : 2) void stage_mfma(S
     if (TP) gather_idx<2, 2>(A, I, a0, 1, threadIdx.x, 256, nbx);
     double2 xr[SPT], xv[SPT];
     // y / acc of this lane's output elements
+    // Y (stages 1-2) and ACC (stage 3) in two arrays: one operand array, as in the other
+    // kernels, measured 0.6 % slower here (148 against 164 VGPRs, 3 waves/SIMD either way)
     double2 yv[TPW][4], av[TPW][4];
     if (TP) {   // every load unconditional: clamped address, selected result
         const double2* yp = (A.stage == 1 || A.stage == 2) ? A.y : A.rin;
@@ -480,9 +480,8 @@ __global__ __launch_bounds__(512, 2) void stage_mfma64(StageArgs A) {
         }
     }
     // y and acc of the lane's 8 output elements (unconditional, clamped), then the update
-    const double2* yp = (st == 1 || st == 2) ? A.y : A.rin;
-    const double2* ap = st == 3 ? A.acc : A.rin;
-    double2 yv[2][4], av[2][4];
+    const double2* wp = st == 3 ? A.acc : ((st == 1 || st == 2) ? A.y : A.rin);
+    double2 wv[2][4];   // Y in stages 1-2, ACC in stage 3
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
         const int col = (tj0 + t) * 16 + li;
@@ -491,8 +490,7 @@ __global__ __launch_bounds__(512, 2) void stage_mfma64(StageArgs A) {
             const int row = ti * 16 + lk + 4 * r;
             const bool in = row < N && col < N;
             const size_t gi = (size_t)(base + (in ? row * N + col : 0));
-            yv[t][r] = yp[gi];
-            av[t][r] = ap[gi];
+            wv[t][r] = wp[gi];
         }
     }
 #pragma unroll
@@ -508,10 +506,9 @@ __global__ __launch_bounds__(512, 2) void stage_mfma64(StageArgs A) {
                 di = fma(g.x, x.y, fma(g.y, x.x, di));
                 const double2 cpl = coupling_of(I, sC, row, col);
                 // two operands: y (x in stage 0) and a (x in stage 1, ACC + x/3 in stage 3)
-                const double2 y0 = st == 0 ? x : yv[t][r];
-                const double2 a1 = st == 1 ? x
-                                 : (st == 3 ? make_double2(fma(x.x, 1.0 / 3.0, av[t][r].x), fma(x.y, 1.0 / 3.0, av[t][r].y))
-                                            : av[t][r]);
+                const double2 w = wv[t][r];
+                const double2 y0 = st == 0 ? x : w;
+                const double2 a1 = st == 1 ? x : (st == 3 ? make_double2(fma(x.x, 1.0 / 3.0, w.x), fma(x.y, 1.0 / 3.0, w.y)) : w);
                 rk_store<false, true>(A, I, (size_t)(base + row * N + col), x, y0, a1, dr + cpl.x, di + cpl.y, a0, row,
                                       col);
             }
