@@ -71,7 +71,8 @@ __device__ __forceinline__ double2 cconj_mul(double2 a, double2 b) {   // conj(a
 }
 
 // adjoint RK stage update of one element (coefficients of the reverse step above), from
-// the stage's input x, lam and ACC (each read where used).  Two stages write no
+// the stage's input x, lam and ACC (each read where used; no stage uses both lam and ACC,
+// and stage 0's lam is its input x).  Two stages write no
 // accumulator: stage 0's output is x1 = h/3 lam + h^2/6 u0, so stage 1 rebuilds
 // lam + h/6 u0 = 2/3 lam + x1/h from its own input; stage 2's output is
 // x3 = h/2 u2 + h/6 lam, so stage 3 adds u2 = 2/h x3 - lam/3 from its own input.  ACC
@@ -268,16 +269,16 @@ __global__ __launch_bounds__(256) void adj_small(AdjArgs J) {
     if (has_cpl) bslot = bath_slot_load(J, I, a0, na, threadIdx.x);
     const int dq = na * I.d;
     const int np_pre = dq > 0 ? J.ado[I.lk_off + (long long)a0 * I.d + (threadIdx.x < dq ? threadIdx.x : 0)] : 0;
-    const double2* lp = (st == 1 || st == 2) ? A.y : A.rin;
-    const double2* ap = st == 3 ? A.acc : A.rin;
-    double2 rv[EPT], lv[EPT], av[EPT], yf[EPT], gv[EPT];
+    // the one state operand of the update besides the input (no stage reads both): lambda
+    // in stages 1-2, ACC in stage 3, the input (lambda itself) in stage 0
+    const double2* wp = st == 3 ? A.acc : ((st == 1 || st == 2) ? A.y : A.rin);
+    double2 rv[EPT], wv[EPT], yf[EPT], gv[EPT];
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
         const int e = threadIdx.x + q * 256;
         const int ec = e < nel ? e : nel - 1;
         rv[q] = A.rin[base + ec];
-        lv[q] = lp[base + ec];
-        av[q] = ap[base + ec];
+        wv[q] = wp[base + ec];
         yf[q] = J.Yf[base + ec];
         gv[q] = A.damp[I.ado_off + a0 + ec / NN];
     }
@@ -352,8 +353,7 @@ __global__ __launch_bounds__(256) void adj_small(AdjArgs J) {
         ur = fma(g.x, r.x, fma(-g.y, r.y, ur));
         ui = fma(g.x, r.y, fma(g.y, r.x, ui));
         const double2 cpl = coupling_of<NT>(I, sC + nl * cpn, ra, cb);
-        const double2 lam = st == 0 ? r : lv[q];
-        rk_store_adj(A, I, (size_t)(base + e), r, lam, av[q], ur + cpl.x, ui + cpl.y);
+        rk_store_adj(A, I, (size_t)(base + e), r, wv[q], wv[q], ur + cpl.x, ui + cpl.y);
     }
     if (has_cpl) reduce_bath_rows<NT>(J, I, a0, na, sR, fs, sU, threadIdx.x, 256, bslot);
     __syncthreads();
@@ -492,9 +492,8 @@ __global__ __launch_bounds__(256) void adj_generic(AdjArgs J) {
         ur = fma(g.x, r.x, fma(-g.y, r.y, ur));
         ui = fma(g.x, r.y, fma(g.y, r.x, ui));
         const double2 cpl = coupling_of(I, sC, ra, cb);
-        const double2 lam = (st == 1 || st == 2) ? A.y[base + ab] : r;
-        const double2 av = st == 3 ? A.acc[base + ab] : r;
-        rk_store_adj(A, I, (size_t)(base + ab), r, lam, av, ur + cpl.x, ui + cpl.y);
+        const double2 w = rk_operand(A, (size_t)(base + ab), r);   // lambda (stages 0-2) or ACC
+        rk_store_adj(A, I, (size_t)(base + ab), r, w, w, ur + cpl.x, ui + cpl.y);
     }
     red[threadIdx.x] = qsum;
     reduce_bath(J, I, a0, 1, sR, NN, fs, sU, threadIdx.x, blockDim.x);
