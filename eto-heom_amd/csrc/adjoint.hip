@@ -357,16 +357,15 @@ __global__ __launch_bounds__(256) void adj_small(AdjArgs J) {
     }
     if (has_cpl) reduce_bath_rows<NT>(J, I, a0, na, sR, fs, sU, threadIdx.x, 256, bslot);
     __syncthreads();
-    // this tile's own accumulator block: no other workgroup touches it in this launch,
-    // and launches are ordered, so a plain (cached) read-modify-write suffices
+    // this tile's own accumulator block: no other workgroup touches it in this launch, so
+    // each entry gets one add per launch in launch order; non-returning atomic adds
+    // (measured 1.2 % faster backward than a read-modify-write, whose read the
+    // workgroup waits for)
     double2* acc = J.slots + J.slot_off[tile];
     const double2* ef = J.field + 3 * (size_t)(I.step_off + step);
     const double2 e0 = ef[0], e1 = ef[1], e2 = ef[2];
     const bool field_on = e0.x != 0.0 || e0.y != 0.0 || e1.x != 0.0 || e1.y != 0.0 || e2.x != 0.0 || e2.y != 0.0;
-    auto add2 = [](double2* p, double2 v) {
-        const double2 o = *p;
-        *p = make_double2(o.x + v.x, o.y + v.y);
-    };
+    auto add2 = [](double2* p, double2 v) { atomic_add2(p, v); };
     for (int ab = threadIdx.x; ab < NN; ab += 256) {
         double2 gs = make_double2(0.0, 0.0);
         for (int n = 0; n < na; ++n) {
