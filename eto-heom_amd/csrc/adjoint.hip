@@ -52,7 +52,7 @@ struct AdjArgs {
     const long long* gacc_off;
     int ncopy;               // tile t adds into copy t % ncopy (spreads the atomics)
     long long traj;          // trajectory mode: Yf moves by step * traj elements
-    double2* slots;          // small-N tiles: one accumulator block per tile (plain read-modify-write)
+    double2* slots;          // small-N tiles: one accumulator block per tile (non-returning atomic adds)
     const long long* slot_off;
 };
 
