@@ -296,7 +296,10 @@ def main():
                      "kernel": "stage (fused RHS + RK4 stage)", "stage_ms": [float(x) for x in prof[1:5]],
                      "prologue_ms": float(prof[0]), "bytes_per_launch": bytes_per_launch,
                      "schedule_bytes_per_launch": schedule_bytes,
-                     "schedule_achieved": schedule_bytes / (stage_ms * 1e-3) / 1e9},
+                     "schedule_achieved": schedule_bytes / (stage_ms * 1e-3) / 1e9,
+                     # the counters' bytes per launch (L2 misses: HBM or the last-level cache)
+                     # over this run's launch time: the rate to quote when the state fits on chip
+                     "traffic_achieved": traffic / (stage_ms * 1e-3) / 1e9 if traffic else None},
     }
     if copy_gbs:
         rec["roofline"]["copy_gbs"] = copy_gbs
