@@ -216,7 +216,7 @@ __global__ __launch_bounds__(256) void stage_generic(StageArgs A) {
 // unconditional) -- a launch candidate offered when every instance of the group has
 // 2 K N <= 512 and K_m <= 2.
 template <int NP, bool VLDS, bool TP = false>
-__global__ __launch_bounds__(256, NP == 32 ? (TP ? 3 : 4) : 2) void stage_mfma(StageArgs A) {
+__global__ __launch_bounds__(256, NP == 32 ? 4 : 2) void stage_mfma(StageArgs A) {
     constexpr int LD = NP + 1;
     constexpr int NB = NP / 16;
     constexpr int TPW = NB * NB / 4;
@@ -247,13 +247,12 @@ __global__ __launch_bounds__(256, NP == 32 ? (TP ? 3 : 4) : 2) void stage_mfma(S
     NbIdx<2, 2> nbx;
     if (TP) gather_idx<2, 2>(A, I, a0, 1, threadIdx.x, 256, nbx);
     double2 xr[SPT], xv[SPT];
-    // y / acc of this lane's output elements
-    // Y (stages 1-2) and ACC (stage 3) in two arrays: one operand array, as in the other
-    // kernels, measured 0.6 % slower here (148 against 164 VGPRs, 3 waves/SIMD either way)
+    // y / acc of this lane's output elements.  The two-phase variant loads the update's
+    // one state operand (Y in stages 1-2, ACC in stage 3) after the commutator instead,
+    // into both arrays: 115 instead of 164 VGPRs, four workgroups per CU instead of three
+    // (measured 6 % faster per RK4 step on the heptamer)
     double2 yv[TPW][4], av[TPW][4];
     if (TP) {   // every load unconditional: clamped address, selected result
-        const double2* yp = (A.stage == 1 || A.stage == 2) ? A.y : A.rin;
-        const double2* ap = A.stage == 3 ? A.acc : A.rin;
 #pragma unroll
         for (int q = 0; q < SPT; ++q) {
             const int e = threadIdx.x + q * 256;
@@ -265,18 +264,6 @@ __global__ __launch_bounds__(256, NP == 32 ? (TP ? 3 : 4) : 2) void stage_mfma(S
             if (VLDS) {
                 const double2 u = V[ix];
                 xv[q] = in ? u : make_double2(0.0, 0.0);
-            }
-        }
-#pragma unroll
-        for (int t = 0; t < TPW; ++t) {
-            const int tt = wave * TPW + t, ti = tt / NB, tj = tt - (tt / NB) * NB;
-            const int col = tj * 16 + li;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = ti * 16 + lk + 4 * r;
-                const size_t gi = (size_t)(base + (row < N && col < N ? row * N + col : 0));
-                yv[t][r] = yp[gi];
-                av[t][r] = ap[gi];
             }
         }
         gather_data<2, 2>(A, I, 1, threadIdx.x, 256, nbx, sC);
@@ -362,6 +349,16 @@ __global__ __launch_bounds__(256, NP == 32 ? (TP ? 3 : 4) : 2) void stage_mfma(S
             }
         }
         const int col = tj * 16 + li;
+        if (TP) {   // the update's one state operand (clamped address), after the commutator
+            const double2* wp = A.stage == 3 ? A.acc : ((A.stage == 1 || A.stage == 2) ? A.y : A.rin);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = ti * 16 + lk + 4 * r;
+                const double2 w = wp[(size_t)(base + (row < N && col < N ? row * N + col : 0))];
+                yv[t][r] = w;
+                av[t][r] = w;
+            }
+        }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int row = ti * 16 + lk + 4 * r;
