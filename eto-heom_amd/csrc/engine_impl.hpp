@@ -57,6 +57,7 @@ struct LaunchCfg {
     int ge;                   // stage_small: state elements per tile (256, 512 or 1024)
     bool cols;                // keep coupled-column copies (contiguous column gathers)
     bool stream;              // non-temporal state stores (StageArgs::stream)
+    bool late = false;        // stage_small<N, 512, true>: state operand loaded after the barrier
     int tile_begin, tile_count;
     size_t shmem;
 };

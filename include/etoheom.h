@@ -164,7 +164,8 @@ int eh_batch_profile(eh_batch* b, int nsteps, double* out5);
 int eh_batch_run_eager(eh_batch* b);
 /* autotuner: time `steps` RK4 steps of every launch candidate of every N class
  * (small-N tile sizes of 256/512/1024 state elements, MFMA / pipelined / generic
- * kernels, the small-N base one with the other state-store cache hint), keep the fastest and
+ * kernels, the small-N base one with the other state-store cache hint and the other load
+ * placement), keep the fastest and
  * re-capture the graph; call before
  * eh_batch_enable_adjoint.  eh_batch_tuning_report writes one line per candidate
  * ("... ms_per_step=<t>", '*' marks the chosen one) into buf (NUL-terminated,

@@ -94,6 +94,8 @@ def test_every_launch_candidate_matches_oracle(ctx, refs):
     rep = b.tuning_report().splitlines()
     for g in {g for g, _, k, _ in cands if k == "stage_small"}:
         assert {l.split("stream=")[1][0] for l in rep if int(l.split()[1]) == g} == {"0", "1"}
+        # and with the state operand loaded with the input or after the tile's barrier
+        assert {l.split("late=")[1][0] for l in rep if int(l.split()[1]) == g} == {"0", "1"}
     for g, c, kern, tile in cands:
         print(f"group {g} candidate {c}: {kern} tile={tile}", flush=True)
         b.select(g, c)
