@@ -122,3 +122,40 @@ def test_autotune_picks_one_candidate_per_group_and_stays_correct(ctx, refs):
     b.run()
     b.sync()
     check(b, items, ref)
+
+
+def test_store_hint_and_load_placement_variants_are_bitwise_equal(ctx, refs):
+    """The small-N base kernel with the non-temporal store hint and/or the state operand
+    loaded after the barrier runs the same arithmetic: every readout and final state is
+    bitwise that of the plain variant (and within 1e-10 of the oracle)."""
+    items, ref = refs
+    b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)} for p in items])
+    rep = b.tuning_report().splitlines()
+    small = sorted({int(l.split()[1]) for l in rep if " stage_small " in l and "tile=512" in l})
+    assert small
+
+    def pick(g, stream, late):
+        for l in rep:
+            f = l.split()
+            if (int(f[1]) == g and " stage_small " in l and "tile=512" in l and f"stream={stream}" in l
+                    and f"late={late}" in l):
+                return int(f[3])
+        return None
+
+    results = {}
+    for stream, late in ((0, 0), (1, 0), (0, 1), (1, 1)):
+        chosen = [(g, pick(g, stream, late)) for g in small]
+        if any(c is None for _, c in chosen):
+            continue
+        for g, c in chosen:
+            b.select(g, c)
+        b.run()
+        b.sync()
+        results[(stream, late)] = ([b.output(i) for i in range(len(items))], [b.rho(i) for i in range(len(items))])
+    assert (0, 0) in results and len(results) >= 3, sorted(results)
+    check(b, items, ref)
+    base_out, base_rho = results[(0, 0)]
+    for key, (outs, rhos) in results.items():
+        for i in range(len(items)):
+            assert np.array_equal(outs[i], base_out[i]), (key, i)
+            assert np.array_equal(rhos[i], base_rho[i]), (key, i)
