@@ -54,7 +54,12 @@ struct AdjArgs {
     long long traj;          // trajectory mode: Yf moves by step * traj elements
     double2* slots;          // small-N tiles: one accumulator block per tile (non-returning atomic adds)
     const long long* slot_off;
+    // readout terms, folded into stage 3 of the workgroup holding ADO 0
+    const int* pidx;         // per step readout slot (-1: none)
+    const double2* R;        // readout operators (X3)
+    const double2* gP;       // dL/dP per readout
 };
+
 
 // layout of one instance's accumulator block (complex entries)
 __host__ __device__ inline long long gacc_size(int NN, int d) { return 5LL * NN + 4LL * d; }
@@ -68,6 +73,27 @@ __device__ __forceinline__ void atomic_add2(double2* p, double2 v) {
 
 __device__ __forceinline__ double2 cconj_mul(double2 a, double2 b) {   // conj(a) * b
     return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+
+// Stage 3 of step i ends with lambda_i, to which the readout P_i = Tr(rho_0 X3) before
+// step i adds dL/dlambda_0[j][k] = conj(X3[k][j]) g_i, g_i = dL/dP_i (and dL/dX3[k][j]
+// gets conj(rho_0[j][k]) g_i from the forward state of step i, this stage's Yf).  Only
+// 2DES instances (one readout operator) read out P.
+__device__ __forceinline__ bool readout_g(const AdjArgs& J, const InstDev& I, int step, double2& g) {
+    const int slot = I.nread == 1 ? J.pidx[I.step_off + step] : -1;
+    if (slot < 0) return false;
+    g = J.gP[I.p_off + slot];
+    return true;
+}
+
+// the readout term of ADO 0's element (j, k) into the stage-3 update u, and its dL/dX3 term
+__device__ __forceinline__ void readout_fold(const AdjArgs& J, const InstDev& I, int ii, int j, int k, double2 rho,
+                                             double2 g, double& ur, double& ui) {
+    const int N = I.N;
+    const double2 t = cconj_mul(J.R[I.r_off + k * N + j], g);
+    ur += t.x;
+    ui += t.y;
+    atomic_add2(J.gacc + J.gacc_off[ii] + 4LL * N * N + k * N + j, cconj_mul(rho, g));
 }
 
 // adjoint RK stage update of one element (coefficients of the reverse step above), from
@@ -311,6 +337,8 @@ __global__ __launch_bounds__(256) void adj_small(AdjArgs J) {
     __syncthreads();
     const double fs = st == 0 ? I.h / 6.0 : 1.0;   // kb = fs * stage input
     const int cpn = 2 * I.K * NT;
+    double2 rg = make_double2(0.0, 0.0);
+    const bool rdo = st == 3 && a0 == 0 && readout_g(J, I, step, rg);
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
         const int e = threadIdx.x + q * 256;
@@ -353,6 +381,7 @@ __global__ __launch_bounds__(256) void adj_small(AdjArgs J) {
         ur = fma(g.x, r.x, fma(-g.y, r.y, ur));
         ui = fma(g.x, r.y, fma(g.y, r.x, ui));
         const double2 cpl = coupling_of<NT>(I, sC + nl * cpn, ra, cb);
+        if (rdo && nl == 0) readout_fold(J, I, ii, ra, cb, Y[ab], rg, ur, ui);
         rk_store_adj(A, I, (size_t)(base + e), r, wv[q], wv[q], ur + cpl.x, ui + cpl.y);
     }
     if (has_cpl) reduce_bath_rows<NT>(J, I, a0, na, sR, fs, sU, threadIdx.x, 256, bslot);
@@ -455,6 +484,8 @@ __global__ __launch_bounds__(256) void adj_generic(AdjArgs J) {
     const double2 e0 = ef[0], e1 = ef[1], e2 = ef[2];
     const bool field_on = e0.x != 0.0 || e0.y != 0.0 || e1.x != 0.0 || e1.y != 0.0 || e2.x != 0.0 || e2.y != 0.0;
     double2 qsum = make_double2(0.0, 0.0);
+    double2 rg = make_double2(0.0, 0.0);
+    const bool rdo = st == 3 && a0 == 0 && readout_g(J, I, step, rg);
     for (int ab = threadIdx.x; ab < NN; ab += blockDim.x) {
         const int ra = ab / N, cb = ab - ra * N;
         double cr = 0.0, ci = 0.0;
@@ -491,6 +522,7 @@ __global__ __launch_bounds__(256) void adj_generic(AdjArgs J) {
         ur = fma(g.x, r.x, fma(-g.y, r.y, ur));
         ui = fma(g.x, r.y, fma(g.y, r.x, ui));
         const double2 cpl = coupling_of(I, sC, ra, cb);
+        if (rdo) readout_fold(J, I, ii, ra, cb, sY[ab], rg, ur, ui);
         const double2 w = rk_operand(A, (size_t)(base + ab), r);   // lambda (stages 0-2) or ACC
         rk_store_adj(A, I, (size_t)(base + ab), r, w, w, ur + cpl.x, ui + cpl.y);
     }
@@ -515,70 +547,34 @@ __global__ __launch_bounds__(256) void adj_generic(AdjArgs J) {
     }
 }
 
-struct AdjBetweenArgs {
+struct AdjVhArgs {
     const InstDev* inst;
     const int* step_ctr;
-    int readout;           // the readout terms of step `step_off`
-    int step_off;
-    int vh;                // V^H of step `vh_step_off`
-    int vh_step_off;
-    const int* pidx;
-    const double2* R;      // readout operators (X3)
-    const double2* gP;     // dL/dP per readout
-    const double2* y;      // y_i (the trajectory slot of this step)
-    double2* lam;
-    double2* gacc;
-    const long long* gacc_off;
-    long long traj;        // trajectory mode: y moves by step * traj elements
     const double2* H;      // V of a step from H, X'_q and the field (as the forward builds it)
     const double2* Xp;
     const double2* field;
-    double2* VH;
+    double2* VH;           // C copies, copy k for step *step_ctr + k
+    long long vstride;     // complex elements per copy
 };
 
-// Between the adjoint stages of step i and those of step i-1, one workgroup per
-// instance: the readout terms of step i
-//   l[ADO 0][j][k] += conj(X3[k][j]) g_i ;  dL/dX3[k][j] += conj(rho_0[j][k]) g_i
-// and V_{i-1}^H = conj(V_{i-1})^T for the next step's adjoint stages (V built from H,
-// X'_q and the field exactly as the forward builds it).  The two touch different data.
-__global__ __launch_bounds__(256) void adj_between(AdjBetweenArgs P) {
-    const int ii = blockIdx.x;
+// V_i^H = conj(V_i)^T of the C steps of a backward segment, one launch per segment
+// (workgroup (instance, k) builds step *step_ctr + k), for the adjoint stages of those
+// steps; V is built from H, X'_q and the field exactly as the forward builds it.
+__global__ __launch_bounds__(256) void adj_vh(AdjVhArgs P) {
+    const int ii = blockIdx.x, k = blockIdx.y;
     const InstDev& I = P.inst[ii];
     const int N = I.N, NN = N * N;
-    if (P.readout) {
-        const int step = *P.step_ctr + P.step_off;
-        const int slot = (step >= 0 && step < I.nsteps && I.nread == 1) ? P.pidx[I.step_off + step] : -1;
-        if (slot >= 0) {
-            const double2 g = P.gP[I.p_off + slot];
-            double2* acc = P.gacc + P.gacc_off[ii] + 4 * NN;
-            for (int q = threadIdx.x; q < NN; q += blockDim.x) {
-                const int j = q / N, k = q - j * N;
-                const double2 x = P.R[I.r_off + k * N + j];
-                double2 l = P.lam[I.state_off + q];
-                const double2 t = cconj_mul(x, g);
-                l.x += t.x;
-                l.y += t.y;
-                P.lam[I.state_off + q] = l;
-                const double2 r = P.y[(long long)step * P.traj + I.state_off + q];
-                const double2 u = cconj_mul(r, g);
-                acc[k * N + j].x += u.x;
-                acc[k * N + j].y += u.y;
-            }
-        }
-    }
-    if (P.vh) {
-        const int step = *P.step_ctr + P.vh_step_off;
-        if (step >= 0 && step < I.nsteps) {
-            const double2* e = P.field + 3 * (size_t)(I.step_off + step);
-            const double2 e0 = e[0], e1 = e[1], e2 = e[2];
-            const double2* H = P.H + I.v_off;
-            const double2* X = P.Xp + 3 * (size_t)I.v_off;
-            for (int q = threadIdx.x; q < NN; q += blockDim.x) {
-                const int a = q / N, b = q - a * N;
-                const double2 v = v_elem(H, X, NN, e0, e1, e2, b * N + a);
-                P.VH[I.v_off + q] = make_double2(v.x, -v.y);
-            }
-        }
+    const int step = *P.step_ctr + k;
+    if (step < 0 || step >= I.nsteps) return;
+    const double2* e = P.field + 3 * (size_t)(I.step_off + step);
+    const double2 e0 = e[0], e1 = e[1], e2 = e[2];
+    const double2* H = P.H + I.v_off;
+    const double2* X = P.Xp + 3 * (size_t)I.v_off;
+    double2* out = P.VH + (long long)k * P.vstride + I.v_off;
+    for (int q = threadIdx.x; q < NN; q += blockDim.x) {
+        const int a = q / N, b = q - a * N;
+        const double2 v = v_elem(H, X, NN, e0, e1, e2, b * N + a);
+        out[q] = make_double2(v.x, -v.y);
     }
 }
 
@@ -752,7 +748,7 @@ int batch_enable_adjoint(Batch* b, int C) {
                   : ((rc = a->ckpt.alloc(T * (size_t)(a->nseg + 1))) || (rc = a->stg.alloc(T * 4 * (size_t)(a->C + 1))) ||
                      (rc = a->acc_f.alloc(T)))) ||
         (rc = a->lam.alloc(T)) || (rc = a->aacc.alloc(T)) || (rc = a->q1.alloc(T)) || (rc = a->q2.alloc(T)) ||
-        (rc = a->VH.alloc(im->V.n / 2)) || (rc = a->gP.alloc(im->P.n)) || (rc = a->gacc.alloc((size_t)gtot)) ||
+        (rc = a->VH.alloc((size_t)a->C * (im->V.n / 2))) || (rc = a->gP.alloc(im->P.n)) || (rc = a->gacc.alloc((size_t)gtot)) ||
         (rc = a->slots.alloc((size_t)stot)) || (rc = a->slot_off.upload(a->h_slot_off)))
         return rc;
     const int blocks = (int)std::min<long long>(std::max<long long>(1, ((long long)T + 255) / 256), 65536);
@@ -819,21 +815,18 @@ int batch_enable_adjoint(Batch* b, int C) {
     aj.ncopy = a->ncopy;
     aj.slots = a->slots.p;
     aj.slot_off = a->slot_off.p;
-    AdjBetweenArgs ar;
+    aj.pidx = im->pidx.p;
+    aj.R = im->R.p;
+    aj.gP = a->gP.p;
+    AdjVhArgs ar;
     std::memset(&ar, 0, sizeof(ar));
     ar.inst = im->inst.p;
     ar.step_ctr = im->step_ctr.p;
-    ar.pidx = im->pidx.p;
-    ar.R = im->R.p;
-    ar.gP = a->gP.p;
-    ar.lam = a->lam.p;
-    ar.gacc = a->gacc.p;
-    ar.gacc_off = a->gacc_off.p;
-    ar.traj = a->trajm ? 4 * (long long)T : 0;
     ar.H = im->H.p;
     ar.Xp = im->Xp.p;
     ar.field = im->field.p;
     ar.VH = a->VH.p;
+    ar.vstride = (long long)(im->V.n / 2);
     aj.traj = a->trajm ? 4 * (long long)T : 0;
     pa.readout = 0;   // replays must not overwrite the forward readouts
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -859,13 +852,13 @@ int batch_enable_adjoint(Batch* b, int C) {
                 }
         }
     }
-    ar.readout = 0;   // V^H of the segment's last step
-    ar.vh = 1;
-    ar.vh_step_off = a->C - 1;
-    hipLaunchKernelGGL(adj_between, dim3(b->n_inst), dim3(256), 0, s, ar);
+    // V^H of every step of the segment (copy k for step k), then the steps in reverse; the
+    // readout terms of step k are folded into its adjoint stage 3
+    hipLaunchKernelGGL(adj_vh, dim3(b->n_inst, a->C), dim3(256), 0, s, ar);
     for (int k = a->C - 1; k >= 0; k--) {
         for (int st = 0; st < 4; st++) {
             AdjArgs x = aj;
+            x.S.V = a->VH.p + (size_t)k * (size_t)ar.vstride;
             x.S.step_off = k;
             x.S.stage = st;
             x.S.y = a->lam.p;
@@ -880,12 +873,6 @@ int batch_enable_adjoint(Batch* b, int C) {
                     return rc;
                 }
         }
-        ar.readout = 1;   // step k's readout terms, and V^H of step k-1
-        ar.step_off = k;
-        ar.y = a->trajm ? tslot(0) : stg(k, 0);
-        ar.vh = k > 0;
-        ar.vh_step_off = k - 1;
-        hipLaunchKernelGGL(adj_between, dim3(b->n_inst), dim3(256), 0, s, ar);
     }
     hipLaunchKernelGGL(tick, dim3(1), dim3(1), 0, s, im->step_ctr.p, -a->C);
     HIPCHK(hipStreamEndCapture(s, &a->gbwd));
