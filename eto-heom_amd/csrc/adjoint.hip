@@ -763,12 +763,19 @@ int batch_enable_adjoint(Batch* b, int C) {
         pa.traj = 4 * (long long)T;
     }
     // forward: C fused steps (readouts in stage 0, the next step's V in stage 3, as the
-    // forward graph does) + (checkpoint store) + tick
+    // forward graph does; with the persistent pipelined kernel, which has no fused tail,
+    // a prologue launch per step instead) + (checkpoint store) + tick
+    bool fused = true;
+    for (const Group& g : im->groups) fused = fused && g.cand[(size_t)g.sel].nt != -33;
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < a->C; k++) {
+        if (!fused) {
+            pa.step_off = k;
+            hipLaunchKernelGGL(prologue, dim3(b->n_inst), dim3(256), 0, s, pa);
+        }
         for (int st = 0; st < 4; st++) {
             StageArgs x = stage_io(fw, im, k, st);
-            x.fused = 1;
+            x.fused = fused ? 1 : 0;
             if (a->trajm) {   // Y_i, S1, S2, S1' of step i in slots 4i .. 4i+3; Y_{i+1} in 4i+4
                 x.y = tslot(0);
                 x.rin = tslot(st);
