@@ -176,6 +176,36 @@ def test_adjoint_generic_large_n(ctx):
         assert abs(got - ref) / max(abs(ref), 1e-12) < 1e-6, (name, idx, got, ref)
 
 
+def test_adjoint_with_pipelined_forward_candidate(ctx):
+    """The persistent pipelined N <= 32 kernel has no fused step tail: with it selected, the
+    adjoint's forward runs unfused steps (a prologue per step) and the gradient agrees with
+    the default candidate's to rounding."""
+    p = configs.heptamer(L=1, nsteps=40)
+    p.pulses[:, 1] = [2.0, 6.0, 10.0]
+    p.pulses[:, 2] = 3.0
+    p.pulses[:, 0] *= 3e5
+    P0, _ = O.twod(oproblem(p), nv=(1, 1, 1))
+    rng = np.random.default_rng(4)
+    w = rng.normal(size=len(P0)) + 1j * rng.normal(size=len(P0))
+    res = []
+    for kern in (None, "stage_mfma_pipe"):
+        b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)}])
+        if kern:
+            c = [c for g, c, k, _ in b.candidates() if k == kern]
+            if not c:
+                pytest.skip("no pipelined candidate (MFMA kernels switched off)")
+            b.select(0, c[0])
+        b.enable_adjoint(8)
+        b.run()
+        b.sync()
+        assert np.max(np.abs(b.output(0) - P0)) / np.max(np.abs(P0)) < 1e-10, kern
+        b.backward([w])
+        res.append(b.grad(0))
+    for name in res[0]:
+        a, c = res[0][name], res[1][name]
+        assert np.max(np.abs(a - c)) <= 1e-9 * np.max(np.abs(a)), name
+
+
 def test_adjoint_chain_n56(ctx):
     """N = 56, the 10-site chain's system (32 < N <= 64: the stage_mfma64 forward, the
     generic adjoint kernel): site energies, a nearest-neighbour coupling, a
