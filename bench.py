@@ -1,19 +1,29 @@
 """Throughput of the FP64 ETO-HEOM RK4 propagation on one MI355X.
 
-A bench "step" is one whole propagation of the workload (every RK4 step of it,
-V(t) built and P(t) read out on the device), with the hierarchy state resident
-in HBM.  The default workload is the heptamer: ground + 7 singles + 21 doubles
-(N=29), 2 ETO modes/site (d=14), depth L=4 (3060 ADOs), 2000 RK4 steps of
-0.5 fs, 3 Gaussian pulses; `value` is ADO-updates/s (one ADO advanced one full
-RK4 step).  Other workloads: --workload dimer | grid2d | chain10 | chain_grid | trimer | trimer_grad.
+The default workload is the largest single-GPU configuration: the 10-site chain
+(ground + 10 singles + 45 doubles, N=56), 2 ETO modes/site (d=20), depth L=3
+(1771 ADOs), as a batch of 64 t1 x 4 t2 = 256 propagations of 3000 RK4 steps of
+0.5 fs (about 91 GB of hierarchy state), each with its three Gaussian pulses.
+
+A bench "step" is one twentieth of the workload's propagation: the next
+S = ceil(max_steps / 20) RK4 steps of every instance (one replay of the forward
+graph captured with S steps; V(t) built and P(t) read out on the device), with
+the state resident in HBM.  The timed region starts from a reset, so 20 timed
+steps are exactly one whole propagation of the batch, pulse windows and every
+readout included; more steps wrap round (reset, step 0 again).  `value` is
+ADO-updates/s (one ADO advanced one full RK4 step).  Other workloads:
+--workload heptamer | dimer | grid2d | chain10 | trimer | trimer_grad.
 
 Printed as one JSON line with the roofline of the dominant kernel (the fused
-RHS+RK4 stage kernel, timed per launch with HIP events on its own stream) and
-the CPU oracle timed on this host over a bounded sample of the same workload.
+RHS+RK4 stage kernel, timed per launch with HIP events on its stream) and the
+CPU oracle timed on this host over a bounded sample of the same workload, one
+process per core.
 """
 import argparse
 import json
+import math
 import os
+import subprocess
 import sys
 import time
 
@@ -39,6 +49,10 @@ def workload(name, nsteps=None):
     if name == "chain10":
         return [configs.chain()], "10-site chain N=56, d=20, L=3, 1771 ADOs, 3000 RK4 steps"
     if name == "chain_grid":
+        return (configs.chain_grid(n_t1=64),
+                "10-site chain N=56 (ground+10+45), d=20, L=3, 1771 ADOs, 3000 RK4 steps dt=0.5 fs: the 64 t1 "
+                "(0..630 fs) x 4 t2 (0/100/200/400 fs) grid, 256 propagations in one batch")
+    if name == "chain_grid16":
         return (configs.chain_grid(n_t1=4),
                 "10-site chain N=56, L=3, 1771 ADOs, 3000 RK4 steps: 16 propagations (t1 = 0..30 fs x t2 = "
                 "0/100/200/400 fs) of the 64 x 4 grid in one batch")
@@ -50,48 +64,92 @@ def workload(name, nsteps=None):
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(p, rho, seconds=20.0, max_threads=16):
-    """The oracle (a scalar FP64 port of the reference CPU path, with its dense
-    one-hot GEMMs and hashed neighbour lookups) on this host's cores, one
-    independent propagation per thread, as the reference farms one process per
-    (tau, T) point.  It starts from the GPU's final hierarchy state, so every ADO is
-    dense as in the bulk of a propagation (the first steps, before the pulses, have
-    an all-zero hierarchy that the reference's zero-skipping GEMM passes over), and
-    runs whole RK4 steps after the last pulse.  A one-thread pass sizes the sample
-    (about seconds/3), then every thread runs that many steps at once."""
-    import threading
+def first_instance(name):
+    """The workload's first propagation, built alone (the CPU sample's instance)."""
+    from etoheom_amd import configs
+    if name == "chain_grid" or name == "chain_grid16":
+        return configs.chain(t1=0.0, t2=0.0)
+    items, _ = workload(name)
+    return items[0]
+
+
+def cpu_worker(name, seconds, seed):
+    """One process of the CPU baseline: the oracle (a scalar FP64 port of the
+    reference CPU path, with its dense one-hot GEMMs and hashed neighbour lookups)
+    on the workload's first instance from a dense random hierarchy state (the
+    reference's GEMM skips zero entries, so the all-zero hierarchy before the
+    pulses would understate its cost), after the last pulse.  Waits for a line on
+    stdin, then doubles its step count from 1 until a run takes seconds/3, and
+    prints that run's steps and time."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
     import oracle as O
+    p = first_instance(name)
     q = O.Problem(H=p.H, X=p.X, alpha=p.alpha, alpha_t=p.alpha_t, gamma=p.gamma, L=p.L,
                   site_offset=p.site_offset, pulses=p.pulses, t_start=p.t_end, t_end=p.t_end + 10000.0,
                   step_size=p.step_size, print_step=p.print_step)
-    M = q.num_ado()
-    steps, elapsed = 1, 0.0
+    M, N = q.num_ado(), p.N
+    rng = np.random.default_rng(seed)
+    rho = (rng.standard_normal((M, N, N)) + 1j * rng.standard_normal((M, N, N))) * (1.0 / (N * M))
+    print("ready", flush=True)
+    sys.stdin.readline()
+    steps = 1
     while True:
         t0 = time.perf_counter()
         O.twod(q, nv=(1, 1, 1), rho_init=rho, max_steps=steps)
         elapsed = time.perf_counter() - t0
         if elapsed > seconds / 3 or steps >= 4096:
             break
-        steps *= 4 if elapsed < seconds / 30 else 2
+        steps *= 2
+    print(json.dumps({"steps": steps, "seconds": elapsed, "ados": M}), flush=True)
+    return 0
+
+
+def cpu_baseline(name, seconds=20.0, max_procs=16):
+    """The oracle on this host's cores: one single-threaded process per core of
+    this process's affinity set (at most max_procs), each on its own copy of the
+    workload's first instance, as the reference farms one process per (tau, T)
+    point (script/cpu-script/submit_jobs.sh:59-65).  All processes start together
+    (each waits for a line on stdin); the rate is the sum of their rates."""
     try:
         ncpu = len(os.sched_getaffinity(0))
     except AttributeError:
         ncpu = os.cpu_count() or 1
-    nthr = max(1, min(max_threads, ncpu))
-    # the C oracle keeps no global state and ctypes releases the GIL during the call
-    threads = [threading.Thread(target=O.twod, args=(q,), kwargs=dict(nv=(1, 1, 1), rho_init=rho, max_steps=steps))
-               for _ in range(nthr)]
+    nproc = max(1, min(max_procs, ncpu))
+    procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker", name,
+                               "--cpu-seconds", str(seconds), "--cpu-seed", str(r)],
+                              stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True) for r in range(nproc)]
+    for pr in procs:
+        line = pr.stdout.readline()
+        if line.strip() != "ready":
+            raise RuntimeError(f"CPU worker failed to start: {line!r}")
     t0 = time.perf_counter()
-    for t in threads:
-        t.start()
-    for t in threads:
-        t.join()
+    for pr in procs:
+        pr.stdin.write("go\n")
+        pr.stdin.flush()
+    res = []
+    for pr in procs:
+        out, _ = pr.communicate()
+        if pr.returncode:
+            raise RuntimeError(f"CPU worker exited with {pr.returncode}")
+        res.append(json.loads(out.strip().splitlines()[-1]))
     wall = time.perf_counter() - t0
-    return {"value": nthr * M * steps / wall, "unit": "ADO-updates/s", "cores": nthr, "kind": "port",
-            "single_thread_value": M * steps / elapsed,
-            "sample": f"{nthr} threads x {steps} RK4 steps of the same instance ({M} ADOs) from its dense final "
-                      f"state, {wall:.1f} s wall (one thread alone: {elapsed:.1f} s)"}
+    rate = sum(r["ados"] * r["steps"] / r["seconds"] for r in res)
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    steps = sorted({r["steps"] for r in res})
+    return {"value": rate, "unit": "ADO-updates/s", "cores": nproc, "kind": "port",
+            "single_process_value": res[0]["ados"] * res[0]["steps"] / res[0]["seconds"],
+            "cpu_model": model, "affinity_cpus": ncpu,
+            "sample": f"{nproc} concurrent single-threaded oracle processes, each {'/'.join(map(str, steps))} RK4 "
+                      f"step(s) of the workload's first instance ({res[0]['ados']} ADOs) from a dense random "
+                      f"state after the last pulse; {wall:.1f} s wall"}
 
 
 def per_propagation(ctx, items, k):
@@ -186,13 +244,39 @@ def bench_gradient(a):
     return 0
 
 
+def complex_v_share(items, first, last):
+    """Share of (instance, RK4 step) pairs in steps [first, last) whose V(t) is complex
+    (some pulse on: the commutator then needs twice the MFMA work), from the same
+    float time grid and window rule the library uses."""
+    import numpy as np
+    on = tot = 0
+    for p in items:
+        f = np.float32
+        n = int(np.floor(float(f(f(p.t_end) - f(p.t_start)) / f(p.step_size)))) + 1
+        t = f(p.t_start)
+        for i in range(min(last, n)):
+            if i >= first:
+                tot += 1
+                if any(abs(float(t) - pl[1]) <= 2 * pl[2] for pl in p.pulses):
+                    on += 1
+            t = f(t + f(p.step_size))
+    return on / max(tot, 1)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="heptamer")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="chain_grid")
+    ap.add_argument("--chunk", type=int, default=0,
+                    help="RK4 steps per bench step (0: 50 for chain_grid, else a twentieth of the propagation)")
+    ap.add_argument("--full", action="store_true",
+                    help="time one whole propagation of the batch (from a reset) instead of --steps bench steps")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-seed", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--nsteps", type=int, default=0, help="shorten the propagation (counter runs only)")
     ap.add_argument("--autotune", type=int, default=0, help="autotuner steps per launch candidate (0: off)")
@@ -202,6 +286,8 @@ def main():
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="also run a plain 2 GiB read + 2 GiB write elementwise copy (counter scale)")
     a = ap.parse_args()
+    if a.cpu_worker:
+        return cpu_worker(a.cpu_worker, a.cpu_seconds, a.cpu_seed)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); independent propagations shard "
               "across GPUs with no communication but the final gather of P(t).", file=sys.stderr)
@@ -228,31 +314,55 @@ def main():
     if a.workload == "trimer_grad":
         return bench_gradient(a)
     items, desc = workload(a.workload, a.nsteps)
+    print(f"bench: {desc}", file=sys.stderr, flush=True)
     ctx = E.Context(0)
-    # end to end from host buffers: hierarchy and tables built on the host, uploaded,
-    # graph captured, propagated, readouts copied back (creation is outside `value`)
-    torch.cuda.synchronize()
-    t_e2e = time.perf_counter()
+    t_setup = time.perf_counter()
     batch = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)} for p in items])
-    batch.run()
-    batch.sync()
-    _ = [batch.output(i) for i in range(len(items))]
-    t_e2e = time.perf_counter() - t_e2e
+    max_steps = batch.max_steps
+    S = a.chunk or (min(50, max_steps) if a.workload == "chain_grid" else -(-max_steps // 20))
+    if a.full:
+        S = -(-max_steps // math.ceil(max_steps / S))   # equal replays covering the whole propagation
+    batch.set_chunk(S)
+    t_setup = time.perf_counter() - t_setup
     tuning = batch.autotune(a.autotune).splitlines() if a.autotune else None
-    for _ in range(a.warmup):
-        batch.run()
-        batch.sync()
+    per_inst = [(p.num_ado(), E._nsteps(p.system()[0])) for p in items]
+
+    pos = [max_steps]
+
+    def bench_step():
+        """The next S RK4 steps of every instance (a reset first when the propagation is done)."""
+        if pos[0] >= max_steps:
+            batch.start()
+            pos[0] = 0
+        batch.advance(S)
+        u = sum(M * max(0, min(pos[0] + S, n) - pos[0]) for M, n in per_inst)
+        pos[0] += S
+        return u
+
+    steps = math.ceil(max_steps / S) if a.full else a.steps
+    warm = 0 if a.full else a.warmup
+    for _ in range(warm):
+        bench_step()
+        print(f"bench: warm-up step done (RK4 steps to {pos[0]})", file=sys.stderr, flush=True)
+    if a.full or warm == 0:
+        pos[0] = max_steps   # the timed region starts from a reset
+    first = pos[0] % max_steps if pos[0] < max_steps else 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(a.steps):
-        batch.run()
-    batch.sync()
+    ups = 0
+    for k in range(steps):
+        ups += bench_step()
+        if k % 5 == 4:
+            print(f"bench: timed step {k + 1}/{steps} issued", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    gpu_ms = batch.elapsed_ms()
+    last = pos[0]
+    gpu_ms = batch.elapsed_ms()   # the last bench step, HIP events on the batch stream
+    done = last >= max_steps
     outs = [batch.output(i) for i in range(len(items))]
-    rho_final = batch.rho(0) if not a.no_cpu else None   # dense end-of-run state for the CPU sample
     finite = all(np.all(np.isfinite(o)) for o in outs)
+    timed_range = (f"RK4 steps {first}..{last - 1} of {max_steps}" if steps * S <= max_steps
+                   else f"{steps * S} RK4 steps from step {first}, wrapping round (reset) at {max_steps}")
 
     # per-launch durations of the stage kernel (HIP events on the batch stream)
     prof = batch.profile(a.profile_steps)
@@ -264,41 +374,53 @@ def main():
     bytes_per_launch = 64.0 * elems
     schedule_bytes = 48.0 * elems
     achieved = bytes_per_launch / (stage_ms * 1e-3) / 1e9
+    # the same kernel in the timed graph replays: wall time per stage launch (batches of one
+    # N class, whose four stage launches per step run one after another)
+    n_groups = len({c[0] for c in batch.candidates()})
+    graph_launch_ms = wall * 1e3 / (steps * S * 4) if n_groups == 1 else None
 
     per_prop = per_propagation(ctx, items, a.per_propagation) if a.per_propagation else None
 
     traffic, tsrc = traffic_of(a.workload)
-    ups = batch.ado_updates * a.steps
     value = ups / wall
+    full_ups = sum(M * n for M, n in per_inst)
     rec = {
         "metric": "FP64 ADO-updates/s + 2DES grid points/s at 1e-10 err; adjoint-grad time vs FD",
         "value": value,
         "unit": "ADO-updates/s",
         "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": wall / a.steps * 1e3,
+        "steps": steps,
+        "warmup": warm,
+        "ms_per_step": wall / steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": desc, "propagations_per_step": len(items),
-                   "propagations_per_s": len(items) * a.steps / wall,
-                   "grid_points_per_s": len(items) * a.steps / wall / 3.0,
-                   "gpu_event_ms_last_run": gpu_ms, "device": ctx.name, "finite": finite,
-                   "groups": batch.tuning_report().splitlines(),
-                   "end_to_end_from_host": {"seconds": t_e2e, "ado_updates_per_s": batch.ado_updates / t_e2e,
-                                            "note": "first run incl. host hierarchy/table build, upload, graph "
-                                                    "capture and readout copy-back; before autotuning"}},
+        "config": {"workload": desc, "propagations": len(items),
+                   "bench_step": f"{S} RK4 steps of every propagation (one replay of the forward graph)",
+                   "timed": timed_range + (" (whole propagation)" if first == 0 and done and steps * S == max_steps
+                                           else ""),
+                   "complex_v_share_timed": complex_v_share(items, first, min(last, max_steps)),
+                   "complex_v_share_whole": complex_v_share(items, 0, max_steps),
+                   "propagations_per_s": value / (full_ups / len(items)),
+                   "gpu_event_ms_last_step": gpu_ms, "device": ctx.name, "finite": finite,
+                   "readouts_complete": done, "groups": batch.tuning_report().splitlines(),
+                   "setup_s": t_setup,
+                   "setup_note": "host hierarchy/table build, upload, allocation and graph capture; outside value"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": tsrc,
                      "kernel": "stage (fused RHS + RK4 stage)", "stage_ms": [float(x) for x in prof[1:5]],
+                     "stage_ms_note": f"HIP events around each stage launch over {a.profile_steps} eager steps "
+                                      "from a reset (before the first pulse: real V)",
+                     "graph_launch_ms": graph_launch_ms,
+                     "graph_frac": (bytes_per_launch / (graph_launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+                                    if graph_launch_ms else None),
                      "prologue_ms": float(prof[0]), "bytes_per_launch": bytes_per_launch,
                      "schedule_bytes_per_launch": schedule_bytes,
                      "schedule_achieved": schedule_bytes / (stage_ms * 1e-3) / 1e9,
                      # the counters' bytes per launch (L2 misses: HBM or the last-level cache)
-                     # over this run's launch time: the rate to quote when the state fits on chip
+                     # over this run's launch time
                      "traffic_achieved": traffic / (stage_ms * 1e-3) / 1e9 if traffic else None},
     }
     if copy_gbs:
@@ -308,8 +430,9 @@ def main():
     if per_prop:
         per_prop["batched_over_per_propagation"] = rec["config"]["propagations_per_s"] / per_prop["propagations_per_s"]
         rec["config"]["per_propagation_path"] = per_prop
+    batch.close()
     if not a.no_cpu:
-        rec["cpu_baseline"] = cpu_baseline(items[0], rho_final)
+        rec["cpu_baseline"] = cpu_baseline(a.workload, a.cpu_seconds)
     print(json.dumps(rec))
     return 0
 

@@ -648,6 +648,12 @@ struct AdjState {
 
 void adj_destroy(AdjState* a) { delete a; }
 
+// dynamic LDS of adj_generic for a group: rho and lambda (N^2 each), the coupling
+// rows/columns (2 K N), six d-vectors of bath sums
+static size_t adj_generic_lds(const Group& g) {
+    return (size_t)(2 * g.maxnn + g.maxkn) * sizeof(double2) + (size_t)6 * g.maxd * sizeof(double);
+}
+
 static int launch_adj(const Group& g, const AdjArgs& base, hipStream_t s) {
     AdjArgs a = base;
     a.S.tile_inst += g.tile_begin;
@@ -663,8 +669,7 @@ static int launch_adj(const Group& g, const AdjArgs& base, hipStream_t s) {
 #undef EH_CASE
         default: {
             // every N > 16 group has one ADO per tile
-            const size_t sh = (size_t)(2 * g.maxnn + g.maxkn) * sizeof(double2) + (size_t)6 * g.maxd * sizeof(double);
-            hipLaunchKernelGGL(adj_generic, grid, block, sh, s, a);
+            hipLaunchKernelGGL(adj_generic, grid, block, adj_generic_lds(g), s, a);
             break;
         }
     }
@@ -727,6 +732,15 @@ int batch_enable_adjoint(Batch* b, int C) {
             a->h_slot_off[(size_t)t] = stot;
             stot += gacc_size(I.N * I.N, I.d);
         }
+    }
+    // the N > 16 adjoint kernel stages rho, lambda and the coupling rows/columns in LDS
+    for (const Group& g : im->groups) {
+        if (g.nt >= 1 && g.nt <= 16) continue;
+        const size_t sh = adj_generic_lds(g);
+        if ((long long)sh > b->ctx->lds_per_block)
+            return fail(EH_ESHAPE, "adjoint: an N > 16 instance needs " + std::to_string(sh) +
+                                       " bytes of LDS per workgroup (2 N^2 + 2 K N complex + 48 d); the device has " +
+                                       std::to_string(b->ctx->lds_per_block));
     }
     int rc;
     const size_t T = (size_t)a->total;
@@ -903,11 +917,14 @@ int adj_forward(Batch* b) {
     return EH_OK;
 }
 
-int batch_backward(Batch* b, const double* dP) {
+int batch_backward(Batch* b, const double* dP, int64_t n_dP) {
     if (!b || !b->impl || !b->impl->adj) return fail(EH_EINVAL, "backward: adjoint not enabled");
     BatchImpl* im = b->impl;
     AdjState* a = im->adj;
     if (!im->ran) return fail(EH_EINVAL, "backward before forward run");
+    if (n_dP != (int64_t)im->P.n)
+        return fail(EH_ESHAPE, "backward: dP has " + std::to_string(n_dP) + " entries, the batch's readouts " +
+                                   std::to_string(im->P.n));
     HIPCHK(hipSetDevice(b->ctx->device));
     hipStream_t s = im->stream;
     if (im->P.n) HIPCHK(hipMemcpyAsync(a->gP.p, dP, im->P.n * sizeof(double2), hipMemcpyHostToDevice, s));

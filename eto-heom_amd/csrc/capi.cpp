@@ -229,6 +229,11 @@ int eh_ctx_info(const eh_ctx* ctx, char* name, int cap, int* n_cu, int* lds) {
     return EH_OK;
 }
 
+int eh_ctx_mem_info(eh_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes) {
+    if (!ctx) return fail(EH_EINVAL, "eh_ctx_mem_info: NULL context");
+    return ctx_mem_info(ctx, free_bytes, total_bytes);
+}
+
 int eh_batch_create(eh_ctx* ctx, int n, const eh_instance* inst, eh_batch** out) {
     if (!out) return fail(EH_EINVAL, "eh_batch_create: NULL argument");
     Batch* b = nullptr;
@@ -254,7 +259,7 @@ int eh_batch_get_output(eh_batch* bb, int i, double* out, int64_t cap, int64_t* 
     if (!bb || !bb->b) return fail(EH_EINVAL, "NULL batch");
     Batch* b = bb->b;
     if (i < 0 || i >= b->n_inst) return fail(EH_ERANGE, "instance index out of range");
-    if (!b->synced) {
+    if (!b->synced) {   // fails with EH_EINVAL when the batch has not run
         int rc = batch_sync(b);
         if (rc) return rc;
     }
@@ -310,8 +315,30 @@ int eh_batch_enable_adjoint(eh_batch* b, int checkpoint_every) {
     return b ? batch_enable_adjoint(b->b, checkpoint_every) : fail(EH_EINVAL, "NULL batch");
 }
 
-int eh_batch_backward(eh_batch* b, const double* dP) {
-    return (b && dP) ? batch_backward(b->b, dP) : fail(EH_EINVAL, "NULL argument");
+int eh_batch_backward(eh_batch* b, const double* dP, int64_t n_dP) {
+    return (b && dP) ? batch_backward(b->b, dP, n_dP) : fail(EH_EINVAL, "NULL argument");
+}
+
+int eh_batch_set_chunk(eh_batch* b, int chunk) {
+    return b ? batch_set_chunk(b->b, chunk) : fail(EH_EINVAL, "NULL batch");
+}
+int eh_batch_start(eh_batch* b) { return b ? batch_start(b->b) : fail(EH_EINVAL, "NULL batch"); }
+int eh_batch_advance(eh_batch* b, int nsteps) {
+    return b ? batch_advance(b->b, nsteps) : fail(EH_EINVAL, "NULL batch");
+}
+
+int eh_batch_plan(int n, const eh_instance* inst, int checkpoint_every, int64_t* o) {
+    if (!o) return fail(EH_EINVAL, "eh_batch_plan: NULL argument");
+    BatchPlan p;
+    const int rc = plan_batch(n, inst, checkpoint_every, p);
+    if (rc) return rc;
+    o[0] = p.total_ados;
+    o[1] = p.total_elems;
+    o[2] = p.max_steps;
+    o[3] = p.forward_bytes;
+    o[4] = p.adjoint_replay_bytes;
+    o[5] = p.adjoint_traj_bytes;
+    return EH_OK;
 }
 
 int eh_batch_get_grad(eh_batch* b, int i, double* dH, double* dX4, double* dalpha, double* dalpha_t,

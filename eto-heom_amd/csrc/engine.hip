@@ -885,6 +885,15 @@ int ctx_init(int device, eh_ctx* ctx) {
     return EH_OK;
 }
 
+int ctx_mem_info(eh_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes) {
+    HIPCHK(hipSetDevice(ctx->device));
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    if (free_bytes) *free_bytes = (int64_t)fr;
+    if (total_bytes) *total_bytes = (int64_t)tot;
+    return EH_OK;
+}
+
 static inline double2 d2(cplx c) { return make_double2(c.real(), c.imag()); }
 
 int launch_cfg(const Group& g, const LaunchCfg& c, const StageArgs& base, hipStream_t s) {
@@ -1077,14 +1086,10 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
 
     for (int i = 0; i < n; i++) {
         const eh_instance& q = in[i];
-        if (!q.sys || !q.H) return fail(EH_EINVAL, "instance without system or H");
+        InstPlan ip;   // the host-side checks (sizes, 32-bit in-instance offsets, time grid)
+        if (const int rc0 = check_instance(q, ip)) return rc0;
         const eh_system& sys = *q.sys;
         const int N = sys.N;
-        if (N <= 0 || N > 64) return fail(EH_ESHAPE, "N must be in 1..64");
-        if (q.mode == EH_MODE_2DES && !q.X4) return fail(EH_EINVAL, "2DES instance needs X4");
-        if (q.mode == EH_MODE_DYNAMICS && (q.n_sites < 0 || (q.n_sites > 0 && !q.sites)))
-            return fail(EH_EINVAL, "dynamics instance: bad sites");
-        if (sys.K < 0 || sys.Km < 0 || sys.K + sys.site_offset > N) return fail(EH_ESHAPE, "bath sites exceed N");
         Hierarchy h;
         int rc = build_hierarchy(sys.K * sys.Km, sys.L, h);
         if (rc) return rc;
@@ -1459,6 +1464,52 @@ int batch_run(Batch* b) {
     }
     HIPCHK(hipEventRecord(im->ev1, im->stream));
     im->ran = true;
+    im->started = false;
+    b->synced = false;
+    return EH_OK;
+}
+
+// The forward graph with `chunk` steps per replay (re-captured).
+int batch_set_chunk(Batch* b, int chunk) {
+    if (!b || !b->impl) return fail(EH_EINVAL, "batch_set_chunk: null batch");
+    BatchImpl* im = b->impl;
+    if (im->adj) return fail(EH_EINVAL, "batch_set_chunk: set the chunk before enabling the adjoint");
+    if (chunk <= 0 || chunk > std::max(1, b->max_steps)) return fail(EH_ERANGE, "batch_set_chunk: chunk out of range");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    HIPCHK(hipStreamSynchronize(im->stream));
+    b->chunk = chunk;
+    im->nchunks = (b->max_steps + chunk - 1) / chunk;
+    im->started = false;
+    return capture_forward(b);
+}
+
+// A propagation in pieces: eh_batch_start resets every state; each eh_batch_advance
+// then replays the forward graph for the next nsteps steps (a multiple of the chunk),
+// between HIP events on the batch stream.  Steps past an instance's last do nothing.
+int batch_start(Batch* b) {
+    if (!b || !b->impl) return fail(EH_EINVAL, "batch_start: null batch");
+    BatchImpl* im = b->impl;
+    if (im->adj) return fail(EH_EINVAL, "batch_start: the adjoint needs the checkpointed run (eh_batch_run)");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    launch_reset(b);
+    HIPCHK(hipGetLastError());
+    im->started = true;
+    im->pos = 0;
+    b->synced = false;
+    return EH_OK;
+}
+
+int batch_advance(Batch* b, int nsteps) {
+    if (!b || !b->impl) return fail(EH_EINVAL, "batch_advance: null batch");
+    BatchImpl* im = b->impl;
+    if (!im->started) return fail(EH_EINVAL, "batch_advance: call eh_batch_start first");
+    if (nsteps <= 0 || nsteps % b->chunk) return fail(EH_EINVAL, "batch_advance: steps must be a positive multiple of the chunk");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    HIPCHK(hipEventRecord(im->ev0, im->stream));
+    for (int c = 0; c < nsteps / b->chunk; c++) HIPCHK(hipGraphLaunch(im->exec, im->stream));
+    HIPCHK(hipEventRecord(im->ev1, im->stream));
+    im->pos += nsteps;
+    im->ran = true;
     b->synced = false;
     return EH_OK;
 }
@@ -1466,6 +1517,7 @@ int batch_run(Batch* b) {
 int batch_sync(Batch* b) {
     if (!b || !b->impl) return fail(EH_EINVAL, "batch_sync: null batch");
     BatchImpl* im = b->impl;
+    if (!im->ran) return fail(EH_EINVAL, "batch_sync: the batch has not run (its readouts are undefined)");
     HIPCHK(hipSetDevice(b->ctx->device));
     HIPCHK(hipStreamSynchronize(im->stream));
     b->P_host.resize((size_t)im->P.n);
@@ -1491,6 +1543,7 @@ int batch_profile(Batch* b, int nsteps, double* out) {
     BatchImpl* im = b->impl;
     HIPCHK(hipSetDevice(b->ctx->device));
     launch_reset(b);
+    im->started = false;
     std::vector<hipEvent_t> ev((size_t)6 * nsteps);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     const StageArgs sa = stage_args(im);
@@ -1551,6 +1604,7 @@ int batch_run_eager(Batch* b) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(im->ev1, im->stream));
     im->ran = true;
+    im->started = false;
     b->synced = false;
     return EH_OK;
 }
@@ -1566,6 +1620,7 @@ int batch_autotune(Batch* b, int steps) {
     if (im->adj) return fail(EH_EINVAL, "batch_autotune: tune before enabling the adjoint");
     HIPCHK(hipSetDevice(b->ctx->device));
     launch_reset(b);
+    im->started = false;
     const StageArgs sa = stage_args(im);
     const PrologueArgs pa = prologue_args(im);
     hipLaunchKernelGGL(prologue, dim3(b->n_inst), dim3(256), 0, im->stream, pa);   // V of step 0
@@ -1643,6 +1698,7 @@ std::string batch_tuning_report(const Batch* b) {
 
 int batch_get_rho(Batch* b, int i, double* rho_out) {
     if (!b || i < 0 || i >= b->n_inst || !rho_out) return fail(EH_EINVAL, "batch_get_rho: bad arguments");
+    if (!b->impl->ran) return fail(EH_EINVAL, "batch_get_rho: the batch has not run (its state is undefined)");
     const Batch::Info& f = b->info[(size_t)i];
     HIPCHK(hipSetDevice(b->ctx->device));
     HIPCHK(hipStreamSynchronize(b->impl->stream));

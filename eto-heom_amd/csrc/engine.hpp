@@ -43,12 +43,16 @@ int batch_elapsed_ms(Batch* b, double* ms);
 int batch_get_rho(Batch* b, int i, double* rho_out);
 int batch_profile(Batch* b, int nsteps, double* out5);
 int batch_run_eager(Batch* b);
+int batch_set_chunk(Batch* b, int chunk);
+int batch_start(Batch* b);
+int batch_advance(Batch* b, int nsteps);
 int batch_autotune(Batch* b, int steps);
 int batch_select(Batch* b, int group, int cand);
 std::string batch_tuning_report(const Batch* b);
 int batch_enable_adjoint(Batch* b, int checkpoint_every);
-int batch_backward(Batch* b, const double* dP);
+int batch_backward(Batch* b, const double* dP, int64_t n_dP);
 int batch_get_grad(Batch* b, int i, double* dH, double* dX4, double* dalpha, double* dalpha_t, double* dgamma);
 int ctx_init(int device, eh_ctx* ctx);
+int ctx_mem_info(eh_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
 
 }  // namespace eh

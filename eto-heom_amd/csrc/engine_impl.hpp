@@ -99,7 +99,9 @@ struct BatchImpl {
     hipEvent_t fork_ev = nullptr;
     std::vector<hipEvent_t> join_ev;
     int nchunks = 0;
-    bool ran = false;
+    bool ran = false;         // a run (or an advance) has been issued since creation
+    bool started = false;     // eh_batch_start: states reset, steps run by eh_batch_advance
+    int pos = 0;              // steps run since the last start
     // host copies the adjoint needs to build its tables
     std::vector<InstDev> hinst;
     std::vector<int> hup, hdn;

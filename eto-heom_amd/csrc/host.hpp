@@ -90,4 +90,23 @@ int build_schedule(const eh_system& sys, int mode, const int nv[3], Schedule& s)
 void rng_stream(unsigned long seed, int count, uint32_t* out);
 int draw_samples(const Problem& k, int n_sample, std::vector<cplx>& H, std::vector<cplx>& X);
 
+// Largest system dimension the stage kernels take.
+constexpr int kMaxN = 64;
+
+// One instance as checked on the host (plan.cpp): sizes and step counts.
+struct InstPlan {
+    int N = 0, M = 0, d = 0, nsteps = 0, npoints = 0, nread = 0;
+    int64_t elems = 0, col_elems = 0;   // state elements M N^2, coupled-column elements M K N
+};
+// Every check batch_create makes on an instance before any device work.
+int check_instance(const eh_instance& q, InstPlan& p);
+
+// Device-memory plan of a batch (eh_batch_plan).
+struct BatchPlan {
+    int64_t total_ados = 0, total_elems = 0, col_elems = 0;
+    int max_steps = 0;
+    int64_t forward_bytes = 0, adjoint_replay_bytes = 0, adjoint_traj_bytes = 0;
+};
+int plan_batch(int n, const eh_instance* inst, int checkpoint_every, BatchPlan& out);
+
 }  // namespace eh

@@ -43,7 +43,8 @@ EXPORTS = [
     "eh_batch_sync", "eh_batch_elapsed_ms", "eh_batch_get_output", "eh_batch_get_rho", "eh_batch_dims",
     "eh_batch_ado_updates", "eh_batch_profile", "eh_batch_enable_adjoint", "eh_batch_backward",
     "eh_batch_get_grad", "eh_batch_run_eager", "eh_batch_autotune", "eh_batch_tuning_report",
-    "eh_batch_select",
+    "eh_batch_select", "eh_batch_plan", "eh_ctx_mem_info", "eh_batch_set_chunk", "eh_batch_start",
+    "eh_batch_advance",
 ]
 
 
@@ -115,7 +116,12 @@ def lib():
             "eh_batch_ado_updates": (I64, [VP]),
             "eh_batch_profile": (C.c_int, [VP, C.c_int, VP]),
             "eh_batch_enable_adjoint": (C.c_int, [VP, C.c_int]),
-            "eh_batch_backward": (C.c_int, [VP, VP]),
+            "eh_batch_backward": (C.c_int, [VP, VP, I64]),
+            "eh_batch_plan": (C.c_int, [C.c_int, P(Instance), C.c_int, VP]),
+            "eh_ctx_mem_info": (C.c_int, [VP, P(I64), P(I64)]),
+            "eh_batch_set_chunk": (C.c_int, [VP, C.c_int]),
+            "eh_batch_start": (C.c_int, [VP]),
+            "eh_batch_advance": (C.c_int, [VP, C.c_int]),
             "eh_batch_get_grad": (C.c_int, [VP, C.c_int, VP, VP, VP, VP, VP]),
             "eh_batch_run_eager": (C.c_int, [VP]),
             "eh_batch_autotune": (C.c_int, [VP, C.c_int]),
@@ -298,12 +304,49 @@ class Context:
         _check(lib().eh_ctx_info(self._h, name, 256, C.byref(ncu), C.byref(lds)))
         self.name, self.n_cu, self.lds_per_block = name.value.decode(), ncu.value, lds.value
 
+    def mem_info(self):
+        """(free, total) device memory in bytes."""
+        fr, tot = C.c_int64(0), C.c_int64(0)
+        _check(lib().eh_ctx_mem_info(self._h, C.byref(fr), C.byref(tot)))
+        return fr.value, tot.value
+
     def close(self):
         if getattr(self, "_h", None) and _lib is not None:
             _lib.eh_ctx_destroy(self._h)
             self._h = None
 
     __del__ = close
+
+
+def _instances(items):
+    """(eh_instance array, keep-alive list) for Batch items."""
+    keep = []
+    arr = (Instance * len(items))()
+    for i, it in enumerate(items):
+        s, k = it["problem"].system()
+        H = _c128(it["H"])
+        X = _c128(it["X"]) if it.get("X") is not None else None
+        sites = np.ascontiguousarray(it.get("sites", ()), dtype=np.int32)
+        keep += [s, k, H, X, sites]
+        arr[i].sys = C.pointer(s)
+        arr[i].H = H.ctypes.data
+        arr[i].X4 = _ptr(X)
+        nv = it.get("nv", (1, 1, 1))
+        for q in range(3):
+            arr[i].nv[q] = nv[q]
+        arr[i].mode = it.get("mode", MODE_2DES)
+        arr[i].sites = sites.ctypes.data if sites.size else None
+        arr[i].n_sites = int(sites.size)
+    return arr, keep
+
+
+def batch_plan(items, checkpoint: int = 32) -> dict:
+    """Host-only checks and device-memory plan of a batch (eh_batch_plan): no device needed."""
+    arr, keep = _instances(items)
+    o = np.zeros(6, dtype=np.int64)
+    _check(lib().eh_batch_plan(len(items), arr, int(checkpoint), o.ctypes.data))
+    return {"total_ados": int(o[0]), "total_elems": int(o[1]), "max_steps": int(o[2]),
+            "forward_bytes": int(o[3]), "adjoint_replay_bytes": int(o[4]), "adjoint_traj_bytes": int(o[5])}
 
 
 def _sys_of(key):
@@ -377,24 +420,8 @@ class Batch:
     def __init__(self, ctx: Context, items, adjoint: int = 0):
         """adjoint > 0 enables gradients with a checkpoint every `adjoint` steps."""
         self.ctx = ctx
-        self._keep = []
-        arr = (Instance * len(items))()
         self.items = items
-        for i, it in enumerate(items):
-            s, keep = it["problem"].system()
-            H = _c128(it["H"])
-            X = _c128(it["X"]) if it.get("X") is not None else None
-            sites = np.ascontiguousarray(it.get("sites", ()), dtype=np.int32)
-            self._keep += [s, keep, H, X, sites]
-            arr[i].sys = C.pointer(s)
-            arr[i].H = H.ctypes.data
-            arr[i].X4 = _ptr(X)
-            nv = it.get("nv", (1, 1, 1))
-            for q in range(3):
-                arr[i].nv[q] = nv[q]
-            arr[i].mode = it.get("mode", MODE_2DES)
-            arr[i].sites = sites.ctypes.data if sites.size else None
-            arr[i].n_sites = int(sites.size)
+        arr, self._keep = _instances(items)
         self._arr = arr
         h = C.c_void_p()
         _check(lib().eh_batch_create(ctx._h, len(items), arr, C.byref(h)))
@@ -412,9 +439,20 @@ class Batch:
         _check(lib().eh_batch_enable_adjoint(self._h, int(checkpoint)))
 
     def backward(self, dP):
-        """dP: list of complex arrays (dL/dRe P + i dL/dIm P), one per instance."""
-        flat = np.ascontiguousarray(np.concatenate([np.asarray(g, dtype=np.complex128).ravel() for g in dP]))
-        _check(lib().eh_batch_backward(self._h, flat.ctypes.data))
+        """dP: list of complex arrays (dL/dRe P + i dL/dIm P), one per instance, each
+        shaped like that instance's output."""
+        if len(dP) != self.n_inst:
+            raise ValueError(f"backward: {len(dP)} gradients for {self.n_inst} instances")
+        parts = [np.asarray(g, dtype=np.complex128).ravel() for g in dP]
+        for i, g in enumerate(parts):
+            n = C.c_int64(0)
+            _check(lib().eh_batch_get_output(self._h, i, None, 0, C.byref(n)), allow_nan=True)
+            want = n.value * max(1, len(self.items[i].get("sites", ())) if self.items[i].get("mode", MODE_2DES)
+                                 != MODE_2DES else 1)
+            if g.size != want:
+                raise ValueError(f"backward: instance {i} has {want} readouts, its gradient {g.size}")
+        flat = np.ascontiguousarray(np.concatenate(parts)) if parts else np.zeros(0, np.complex128)
+        _check(lib().eh_batch_backward(self._h, flat.ctypes.data, flat.size))
 
     def grad(self, i: int):
         """Gradients of instance i: dict H (N,N), X (4,N,N), alpha, alpha_t, gamma (K,Km)."""
@@ -457,6 +495,19 @@ class Batch:
         out = np.zeros(5)
         _check(lib().eh_batch_profile(self._h, nsteps, out.ctypes.data))
         return out
+
+    def set_chunk(self, chunk: int):
+        """Re-capture the forward graph with `chunk` RK4 steps per replay."""
+        _check(lib().eh_batch_set_chunk(self._h, int(chunk)))
+        self.chunk = int(chunk)
+
+    def start(self):
+        """Reset every state; then advance() runs the propagation piece by piece."""
+        _check(lib().eh_batch_start(self._h))
+
+    def advance(self, nsteps: int):
+        """Run the next nsteps RK4 steps of every instance (a multiple of the chunk)."""
+        _check(lib().eh_batch_advance(self._h, int(nsteps)))
 
     def run_eager(self):
         """The same propagation with every kernel launched one by one from the host (no graph)."""

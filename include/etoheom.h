@@ -105,6 +105,9 @@ int eh_ctx_create(int device, eh_ctx** out);
 int eh_ctx_destroy(eh_ctx* ctx);
 /* device name / CU count / LDS per block, for logs */
 int eh_ctx_info(const eh_ctx* ctx, char* name, int cap, int* n_cu, int* lds_per_block);
+/* free and total device memory (hipMemGetInfo), for sizing batches with eh_batch_plan.
+ * No reference counterpart: the reference holds one propagation per process. */
+int eh_ctx_mem_info(eh_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
 
 /* ---- one propagation (twoD_spectrum_solver, dynamics.cpp:298-432) ---------
  * X4 = X[0..3] (4*N*N complex).  P_out receives one complex per readout point
@@ -142,13 +145,33 @@ typedef struct eh_instance {
 } eh_instance;
 
 int eh_batch_create(eh_ctx* ctx, int n, const eh_instance* inst, eh_batch** out);
+/* Host-only (no device call): runs every check eh_batch_create makes on the
+ * instances (sizes, at most 2^31 - 1 state elements per instance, time grid) and
+ * returns out6 = {total ADOs, total state elements, max steps, device bytes of the
+ * forward batch, device bytes with the adjoint enabled at `checkpoint_every`
+ * (checkpoint + replay), device bytes with the adjoint's whole trajectory kept}.
+ * No reference counterpart (the reference allocates per ADO inside the solver,
+ * utilize.cpp:30-35, dynamics.cpp:311-341). */
+int eh_batch_plan(int n, const eh_instance* inst, int checkpoint_every, int64_t* out6);
 int eh_batch_destroy(eh_batch* b);
 /* reset every state to rho_0 = |0><0| and run all steps (asynchronous) */
 int eh_batch_run(eh_batch* b);
 int eh_batch_sync(eh_batch* b);
+/* A propagation in pieces: the time loop of twoD_spectrum_solver / dynamics_solver
+ * (dynamics.cpp:385-419, :257-289) split into segments.  eh_batch_start resets
+ * every state to rho_0 = |0><0| (asynchronous); each eh_batch_advance runs the next
+ * nsteps RK4 steps of every instance (a positive multiple of the graph chunk; steps
+ * past an instance's last do nothing), between HIP events (eh_batch_elapsed_ms).
+ * After all steps, outputs and states equal eh_batch_run's bit for bit.
+ * eh_batch_set_chunk re-captures the forward graph with `chunk` steps per replay
+ * (1 <= chunk <= max steps; before eh_batch_enable_adjoint). */
+int eh_batch_set_chunk(eh_batch* b, int chunk);
+int eh_batch_start(eh_batch* b);
+int eh_batch_advance(eh_batch* b, int nsteps);
 /* elapsed GPU time of the last run, from HIP events on the batch stream */
 int eh_batch_elapsed_ms(eh_batch* b, double* ms);
-/* readouts of instance i: 2DES -> complex per point; dynamics -> n_sites x points real */
+/* readouts of instance i: 2DES -> complex per point; dynamics -> n_sites x points real.
+ * EH_EINVAL before the batch has run (its readouts are undefined until then) */
 int eh_batch_get_output(eh_batch* b, int i, double* out, int64_t cap, int64_t* n_points);
 int eh_batch_get_rho(eh_batch* b, int i, double* rho_out);
 /* sizes: out[0..] = {n_inst, total ADOs, total state elements, max steps, graph chunk} */
@@ -178,14 +201,15 @@ int64_t eh_batch_tuning_report(const eh_batch* b, char* buf, int64_t cap);
  * No reference counterpart (the reference has no derivatives); this is the
  * "adjoint gradient" of the propagation path.  After enable, eh_batch_run stores
  * the state every `checkpoint_every` steps; eh_batch_backward takes
- * dP = dL/dRe P + i dL/dIm P for every 2DES readout (instances concatenated in
- * order, eh_batch_get_output's point count each), replays each checkpoint
+ * dP = dL/dRe P + i dL/dIm P for every readout (instances concatenated in
+ * order, eh_batch_get_output's point count times readout operators each; n_dP
+ * complex entries in all, EH_ESHAPE unless that is the batch's total), replays each checkpoint
  * segment and runs the transposed hierarchy backwards.  eh_batch_get_grad then
  * returns dL/dH (N*N), dL/dX (4*N*N, X[0..3] as passed in), dL/dalpha,
  * dL/dalpha_t, dL/dgamma (K*Km) of instance i, same convention, complex
  * interleaved; any output pointer may be NULL. */
 int eh_batch_enable_adjoint(eh_batch* b, int checkpoint_every);
-int eh_batch_backward(eh_batch* b, const double* dP);
+int eh_batch_backward(eh_batch* b, const double* dP, int64_t n_dP);
 int eh_batch_get_grad(eh_batch* b, int i, double* dH, double* dX4, double* dalpha, double* dalpha_t,
                       double* dgamma);
 

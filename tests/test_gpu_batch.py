@@ -159,3 +159,53 @@ def test_store_hint_and_load_placement_variants_are_bitwise_equal(ctx, refs):
         for i in range(len(items)):
             assert np.array_equal(outs[i], base_out[i]), (key, i)
             assert np.array_equal(rhos[i], base_rho[i]), (key, i)
+
+
+def test_propagation_in_pieces_equals_one_run(ctx, refs):
+    """eh_batch_start + eh_batch_advance over every step (another graph chunk) gives the
+    bits of eh_batch_run, and a second start resets the states."""
+    items, ref = refs
+    b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)} for p in items])
+    b.run()
+    b.sync()
+    out0 = [b.output(i) for i in range(len(items))]
+    rho0 = [b.rho(i) for i in range(len(items))]
+    b.set_chunk(7)
+    with pytest.raises(E.EtoHeomError):
+        b.advance(7)           # not started
+    for rep in range(2):
+        b.start()
+        with pytest.raises(E.EtoHeomError):
+            b.advance(5)       # not a multiple of the chunk
+        done = 0
+        while done < b.max_steps:   # the last piece runs past the longest instance's end
+            b.advance(14)
+            done += 14
+        b.sync()
+        for i in range(len(items)):
+            assert np.array_equal(b.output(i), out0[i]), (rep, i)
+            assert np.array_equal(b.rho(i), rho0[i]), (rep, i)
+    check(b, items, ref)
+    b.close()
+
+
+def test_reads_before_a_run_and_misshaped_gradients_are_refused(ctx, refs):
+    items, _ = refs
+    b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)} for p in items[:2]], adjoint=16)
+    with pytest.raises(E.EtoHeomError) as ei:
+        b.output(0)
+    assert ei.value.code == -1
+    with pytest.raises(E.EtoHeomError):
+        b.rho(0)
+    b.run()
+    b.sync()
+    n0, n1 = len(b.output(0)), len(b.output(1))
+    with pytest.raises(ValueError):
+        b.backward([np.ones(n0)])                        # one instance missing
+    with pytest.raises(ValueError):
+        b.backward([np.ones(n0), np.ones(n1 - 1)])       # one readout short
+    # the C ABI checks the length itself
+    flat = np.ones(n0 + n1 - 1, dtype=np.complex128)
+    assert E.lib().eh_batch_backward(b._h, flat.ctypes.data, flat.size) == -3
+    b.backward([np.ones(n0), np.ones(n1)])
+    b.close()

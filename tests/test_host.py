@@ -231,3 +231,46 @@ def test_torch_system_builder_matches_key_path(tmp_path):
     H, X = AG.build_2des_system(torch.tensor(eps), torch.tensor(J), torch.tensor(mu), lam, pol)
     assert np.allclose(H.numpy(), k.construct_hal(), rtol=0, atol=1e-15)
     assert np.allclose(X.numpy(), k.get("X"), rtol=0, atol=1e-15)
+
+
+def _chain_item(L, nsteps=50):
+    """An N = 56 (10-site) 2DES system with 2 modes per bath at depth L, built without a key file."""
+    n = 10
+    N = 1 + n + n * (n - 1) // 2
+    al = np.full((n, 2), 3.3e-5 - 1.5e-5j)
+    p = E.Problem(H=np.diag(np.linspace(0, 1, N)).astype(complex), alpha=al, alpha_t=al.conj(),
+                  gamma=np.full((n, 2), 0.023 + 0j), L=L, X=np.zeros((4, N, N), complex), site_offset=1,
+                  pulses=np.array([[1e-4 / HB, 40.0, 15.0, 2.27]] * 3), t_start=0.0,
+                  t_end=0.5 * (nsteps - 1), step_size=0.5, print_step=0.5)
+    return {"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)}
+
+
+def test_batch_plan_rejects_instances_beyond_32bit_offsets():
+    # 10-site chain at L = 7: 888,030 ADOs x 56^2 = 2.78e9 state elements in one instance,
+    # beyond the kernels' 32-bit in-instance offsets: refused on the host, before any device work
+    with pytest.raises(E.EtoHeomError) as ei:
+        E.batch_plan([_chain_item(7)])
+    assert ei.value.code == -7 and "2^31" in str(ei.value)
+    # L = 6 (230,230 ADOs, 7.2e8 elements) is accepted
+    pl = E.batch_plan([_chain_item(6)])
+    assert pl["total_ados"] == math.comb(26, 6) and pl["total_elems"] == math.comb(26, 6) * 56 * 56
+
+
+def test_batch_plan_bytes():
+    items = [_chain_item(3, nsteps=3000) for _ in range(4)]
+    pl = E.batch_plan(items, checkpoint=32)
+    T = 4 * 1771 * 56 * 56
+    assert pl["total_elems"] == T and pl["max_steps"] == 3000
+    # four state buffers and three coupled-column copies (K N per ADO) dominate the forward
+    state = 16 * (4 * T + 3 * 4 * 1771 * 10 * 56)
+    assert state <= pl["forward_bytes"] < state * 1.01
+    # checkpoints every 32 steps (94 segments + 1), 4 (32 + 1) stage inputs, replay accumulator,
+    # lambda and its three work buffers
+    adj = 16 * T * (95 + 132 + 1 + 4)
+    assert adj <= pl["adjoint_replay_bytes"] < adj * 1.01
+    assert pl["adjoint_traj_bytes"] > 16 * T * (4 * 3000 + 1)
+    # instance checks: N range, bath sites, missing X4
+    bad = _chain_item(1)
+    bad["X"] = None
+    with pytest.raises(E.EtoHeomError):
+        E.batch_plan([bad])
