@@ -273,6 +273,9 @@ def main():
                     help="RK4 steps per bench step (0: 50 for chain_grid, else a twentieth of the propagation)")
     ap.add_argument("--full", action="store_true",
                     help="time one whole propagation of the batch (from a reset) instead of --steps bench steps")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="only the per-launch timing pass (--profile-steps eager steps from a reset); for "
+                         "rocprofv3 runs whose kernel averages must match the roofline figure")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
@@ -325,6 +328,15 @@ def main():
     batch.set_chunk(S)
     t_setup = time.perf_counter() - t_setup
     tuning = batch.autotune(a.autotune).splitlines() if a.autotune else None
+    if a.profile_only:
+        prof = batch.profile(a.profile_steps)
+        elems = sum(p.num_ado() * p.N * p.N for p in items)
+        stage_ms = float(np.mean(prof[1:5]))
+        print(json.dumps({"workload": desc, "profile_steps": a.profile_steps, "prologue_ms": float(prof[0]),
+                          "stage_ms": [float(x) for x in prof[1:5]], "bytes_per_launch": 64.0 * elems,
+                          "achieved_gbs": 64.0 * elems / (stage_ms * 1e-3) / 1e9, "copy_gbs": copy_gbs}))
+        batch.close()
+        return 0
     per_inst = [(p.num_ado(), E._nsteps(p.system()[0])) for p in items]
 
     pos = [max_steps]

@@ -251,3 +251,23 @@ def test_field_window_and_transposition():
     assert np.array_equal(O.field_V(p, 44.0, (1, 0, 0)), H)       # pulse 1 window is [4, 36]
     assert not np.array_equal(O.field_V(p, 44.0, (0, 1, 0)), H)   # pulse 2 window is [20, 52]
     assert np.array_equal(O.field_V(p, 44.0, (1, 0, 0)), O.field_V(p, 44.0, (0, 0, 0)))
+
+
+def test_long_run_fixtures_are_this_oracles_output():
+    """tests/golden/*.npz (made by make_fixtures.py) hold the oracle's output for their own
+    inputs: re-run the trimer fixture's first 500 steps and compare its first readouts bit
+    for bit; check every fixture's sizes."""
+    import os
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    for name, nado, npts in [("trimer_L5_2000", 462, 1620), ("tetramer_L6_2000", 3003, 1620),
+                             ("heptamer_L4_600", 3060, 220)]:
+        f = np.load(os.path.join(gold, name + ".npz"))
+        t = f["time"]
+        q = O.Problem(H=f["H"], X=f["X"], alpha=f["alpha"], alpha_t=f["alpha_t"], gamma=f["gamma"], L=int(f["L"]),
+                      site_offset=int(f["site_offset"]), pulses=f["pulses"], t_start=float(t[0]),
+                      t_end=float(t[1]), step_size=float(t[2]), print_step=float(t[3]))
+        assert q.num_ado() == nado == len(f["ado_norms"]) and len(f["P"]) == npts
+        assert np.max(np.abs(f["P"])) > 0
+        if name == "trimer_L5_2000":
+            P, _ = O.twod(q, nv=tuple(int(v) for v in f["nv"]), max_steps=500)
+            assert len(P) == 120 and np.array_equal(P, f["P"][:120])
