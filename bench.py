@@ -56,6 +56,10 @@ def workload(name, nsteps=None):
         return (configs.chain_grid(n_t1=4),
                 "10-site chain N=56, L=3, 1771 ADOs, 3000 RK4 steps: 16 propagations (t1 = 0..30 fs x t2 = "
                 "0/100/200/400 fs) of the 64 x 4 grid in one batch")
+    if name == "chain_grad":
+        return None, ("10-site chain N=56, d=20, L=3, 1771 ADOs, 3000 RK4 steps: 16 propagations (t1 = 0..30 fs x "
+                      "t2 = 0/100/200/400 fs) of its grid, loss gradient w.r.t. the 10 site energies and 9 couplings "
+                      "by the adjoint (checkpoint every 32 steps, sub-batches sized to the device memory)")
     if name == "trimer_grad":
         return None, ("trimer N=7 (ground+3+3), d=6, L=5, 462 ADOs: 16x16 (t1, t3) rephasing grid at t2=100 fs, "
                       "48 propagations, loss gradient by the adjoint (checkpoint every 32 steps)")
@@ -184,23 +188,41 @@ def traffic_of(workload):
 
 
 def bench_gradient(a):
-    """Forward + adjoint of the trimer 2DES-grid loss; reports the cost ratio
-    (forward + backward) / forward and the equivalent central-FD cost."""
+    """Forward + adjoint gradient of a 2DES-grid loss; reports the cost ratio
+    (forward + backward) / forward and the equivalent central-FD cost.
+    trimer_grad: the trimer's 16 x 16 (t1, t3) rephasing grid (48 propagations), one batch.
+    chain_grad: 16 points of the 10-site chain's (t1, t2) grid (N = 56, L = 3, 3000 steps),
+    gradient w.r.t. all site energies and couplings; the adjoint state of all 16 exceeds
+    the device, so the backward runs in sub-batches (each re-runs its forward with
+    checkpoints every 32 steps, then its backward)."""
     import numpy as np
     import torch
     import etoheom_amd as E
     from etoheom_amd import spectra as SP
     ctx = E.Context(0)
-    spec = SP.grid_spec(ctx)
+    if a.workload == "chain_grad":
+        spec = SP.chain_grid_spec(ctx, t1s=(0.0, 10.0, 20.0, 30.0), t2s=(0.0, 100.0, 200.0, 400.0), nsteps=3000)
+        mk = SP.chain_params
+        sig = SP.chain_grid_signal
+        target_params = SP.chain_params(perturb=7)
+        M, grid_points = 1771, None
+    else:
+        spec = SP.grid_spec(ctx)
+        mk = SP.trimer_params
+        sig = SP.grid_signal
+        target_params = SP.trimer_params(perturb=5)
+        M, grid_points = 462, 16
+    spec.progress = lambda m: print(f"bench: {m}", file=sys.stderr, flush=True)
     with torch.no_grad():
-        target = SP.grid_signal(SP.trimer_params(perturb=5), spec)
+        target = sig(target_params, spec)
+    print("bench: target grid done", file=sys.stderr, flush=True)
     import gc
-    fw, bw, gfw, gbw, gpf = [], [], [], [], []
+    fw, bw, gfw, gbw, gpf, grf = [], [], [], [], [], []
     for it in range(a.warmup + a.steps):
-        params = SP.trimer_params()
+        params = mk()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        S = SP.grid_signal(params, spec)
+        S = sig(params, spec)
         loss = SP.grid_loss(S, target)
         t1 = time.perf_counter()
         loss.backward()
@@ -210,36 +232,44 @@ def bench_gradient(a):
             bw.append(t2 - t1)
             gfw.append(spec.timing["forward_ms"] * 1e-3)
             gbw.append(spec.timing["backward_ms"] * 1e-3)
+            grf.append(spec.timing.get("recompute_forward_ms", 0.0) * 1e-3)
+            subs = spec.timing["sub_batches"]
         lval = float(loss.detach())
         del S, loss   # the graph holds the batch (and its trajectory): free it before the next forward
         gc.collect()
-        with torch.no_grad():   # one plain forward solve (no adjoint state) of the same grid
-            SP.grid_signal(params, spec)
-        if it >= a.warmup:
-            gpf.append(spec.timing["forward_ms"] * 1e-3)
+        if a.workload == "chain_grad":
+            if it >= a.warmup:   # the gradient's forward was already the plain one (no adjoint state)
+                gpf.append(gfw[-1])
+        else:
+            with torch.no_grad():   # one plain forward solve (no adjoint state) of the same grid
+                sig(params, spec)
+            if it >= a.warmup:
+                gpf.append(spec.timing["forward_ms"] * 1e-3)
     n_params = sum(p.numel() for p in params.values())
     f, b = float(np.mean(fw)), float(np.mean(bw))
-    gf, gb, gp = float(np.mean(gfw)), float(np.mean(gbw)), float(np.mean(gpf))
-    # ADO-updates of the forward: 48 propagations x 462 ADOs x their steps
+    gf, gb, gp, gr = float(np.mean(gfw)), float(np.mean(gbw)), float(np.mean(gpf)), float(np.mean(grf))
     steps = sum(int(np.floor(np.float32(np.float32(te) / np.float32(spec.step_size)))) + 1 for te in spec.t_end)
-    ups = 462 * steps
+    ups = M * steps
+    cfg = {"workload": workload(a.workload)[1], "device": ctx.name,
+           "propagations_per_s": len(spec.nv) / gf,
+           "adjoint": {"forward_gpu_s": gf, "backward_gpu_s": gb, "gradient_over_forward": (gf + gb) / gf,
+                       "plain_forward_gpu_s": gp, "gradient_over_plain_forward": (gf + gb) / gp,
+                       "backward_recompute_forward_gpu_s": gr, "sub_batches": subs,
+                       "forward_wall_s": f, "backward_wall_s": b, "gradient_over_forward_wall": (f + b) / f,
+                       "n_params": n_params, "central_fd_s": 2 * n_params * gf,
+                       "fd_over_adjoint": 2 * n_params * gf / (gf + gb), "loss": lval,
+                       "note": "GPU: HIP events around the propagation and the backward pass; "
+                               "forward = the gradient run's forward (with one batch it also keeps the "
+                               "adjoint's state; in sub-batches it is a plain forward and the backward "
+                               "re-runs each sub-batch's forward with checkpoints), plain forward = the "
+                               "same grid with no gradient wanted; wall: incl. host set-up of each batch "
+                               "(tables, graph capture, allocation)"}}
+    if grid_points:
+        cfg["grid_points_per_s"] = grid_points / gf
     rec = {"metric": "FP64 ADO-updates/s + 2DES grid points/s at 1e-10 err; adjoint-grad time vs FD",
            "value": ups / gf, "unit": "ADO-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
            "ms_per_step": (f + b) * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f64", "data": "synthetic",
-           "config": {"workload": workload("trimer_grad")[1], "device": ctx.name,
-                      "grid_points_per_s": 16 / gf,
-                      "adjoint": {"forward_gpu_s": gf, "backward_gpu_s": gb, "gradient_over_forward": (gf + gb) / gf,
-                                  "plain_forward_gpu_s": gp, "gradient_over_plain_forward": (gf + gb) / gp,
-                                  "forward_wall_s": f, "backward_wall_s": b,
-                                  "gradient_over_forward_wall": (f + b) / f,
-                                  "n_params": n_params, "central_fd_s": 2 * n_params * gf,
-                                  "fd_over_adjoint": 2 * n_params * gf / (gf + gb), "loss": lval,
-                                  "note": "GPU: HIP events around the propagation and the backward pass; "
-                                          "forward = the gradient run's forward (it also keeps the adjoint's "
-                                          "trajectory), plain forward = the same grid with no gradient wanted; "
-                                          "wall: incl. host set-up of each batch (tables, graph capture, "
-                                          "allocation)"}}}
+           "dtype": "f64", "data": "synthetic", "config": cfg}
     print(json.dumps(rec))
     return 0
 
@@ -314,7 +344,7 @@ def main():
         torch.cuda.synchronize()
         copy_gbs = 3 * 2 * x.numel() * 8 / (e0.elapsed_time(e1) * 1e-3) / 1e9
         del x, y
-    if a.workload == "trimer_grad":
+    if a.workload in ("trimer_grad", "chain_grad"):
         return bench_gradient(a)
     items, desc = workload(a.workload, a.nsteps)
     print(f"bench: {desc}", file=sys.stderr, flush=True)
@@ -334,7 +364,8 @@ def main():
         stage_ms = float(np.mean(prof[1:5]))
         print(json.dumps({"workload": desc, "profile_steps": a.profile_steps, "prologue_ms": float(prof[0]),
                           "stage_ms": [float(x) for x in prof[1:5]], "bytes_per_launch": 64.0 * elems,
-                          "achieved_gbs": 64.0 * elems / (stage_ms * 1e-3) / 1e9, "copy_gbs": copy_gbs}))
+                          "achieved_gbs": 64.0 * elems / (stage_ms * 1e-3) / 1e9, "copy_gbs": copy_gbs,
+                          "groups": batch.tuning_report().splitlines()}))
         batch.close()
         return 0
     per_inst = [(p.num_ado(), E._nsteps(p.system()[0])) for p in items]

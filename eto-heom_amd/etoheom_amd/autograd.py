@@ -9,6 +9,14 @@ complex-gradient convention (dL/dRe + i dL/dIm), so any differentiable
 construction of those tensors from physical parameters (site energies,
 couplings, dipoles, ETO mode rates) chains through torch as usual.
 
+When the adjoint's device state (checkpoints and a segment's stage inputs, sized by
+the host-side eh_batch_plan) of the whole batch exceeds the device memory
+(`Spec.max_batch_bytes`, default 85 % of the free memory), the batch is split:
+the forward runs as plain batches without adjoint state, and the backward runs each
+sub-batch again with checkpoints and then backwards; every instance's gradient
+depends only on its own propagation, so the sub-batches' gradients are the
+single batch's (to the rounding of the accumulators' atomic adds).
+
 `build_2des_system` is such a construction, written after the reference's
 setup path: the lambda shift of the BATH block (param.cpp:189), construct_Hal's
 one-exciton sort and double-exciton block (dynamics.cpp:24-49), the dipole
@@ -22,7 +30,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from . import MODE_2DES, Batch, Context, Problem
+from . import MODE_2DES, Batch, Context, EtoHeomError, Problem, batch_plan
 
 HBAR = 5308.0
 
@@ -40,58 +48,149 @@ class Spec:
     nv: list = field(default_factory=list)   # (B,) tuples
     checkpoint: int = 32
     site_offset: int = 1
+    max_batch_bytes: int | None = None   # device bytes one batch may take (None: 85 % of the free memory)
+    progress: object = None              # callable(str) told of each finished batch, or None
     timing: dict = field(default_factory=dict)   # filled: GPU-event ms of the last forward / backward
+
+
+def _items(H, X, alpha, alpha_t, gamma, spec: Spec):
+    """Batch items of the B propagations."""
+    items = []
+    for i in range(H.shape[0]):
+        p = Problem(H=H[i].detach().cpu().numpy().astype(np.complex128),
+                    X=X[i].detach().cpu().numpy().astype(np.complex128),
+                    alpha=alpha[i].detach().cpu().numpy().astype(np.complex128),
+                    alpha_t=alpha_t[i].detach().cpu().numpy().astype(np.complex128),
+                    gamma=gamma[i].detach().cpu().numpy().astype(np.complex128),
+                    L=spec.L, site_offset=spec.site_offset, pulses=np.asarray(spec.pulses[i]),
+                    t_start=float(spec.t_start[i]), t_end=float(spec.t_end[i]),
+                    step_size=spec.step_size, print_step=spec.print_step)
+        nv = spec.nv[i] if spec.nv else (1, 1, 1)
+        items.append({"problem": p, "H": p.H, "X": p.X, "nv": tuple(nv), "mode": MODE_2DES})
+    return items
+
+
+def instance_bytes(H, X, alpha, alpha_t, gamma, spec: Spec, adjoint: bool = True):
+    """Device bytes of each propagation alone (eh_batch_plan): its forward state plus,
+    with adjoint, its checkpoint + replay state at spec.checkpoint."""
+    out = []
+    for it in _items(H, X, alpha, alpha_t, gamma, spec):
+        pl = batch_plan([it], checkpoint=spec.checkpoint)
+        out.append(pl["forward_bytes"] + (pl["adjoint_replay_bytes"] if adjoint else 0))
+    return out
 
 
 class _Propagate(torch.autograd.Function):
     @staticmethod
     def forward(ctx, H, X, alpha, alpha_t, gamma, spec: Spec, want_grad: bool):
         B = H.shape[0]
-        items = []
-        for i in range(B):
-            p = Problem(H=H[i].detach().cpu().numpy().astype(np.complex128),
-                        X=X[i].detach().cpu().numpy().astype(np.complex128),
-                        alpha=alpha[i].detach().cpu().numpy().astype(np.complex128),
-                        alpha_t=alpha_t[i].detach().cpu().numpy().astype(np.complex128),
-                        gamma=gamma[i].detach().cpu().numpy().astype(np.complex128),
-                        L=spec.L, site_offset=spec.site_offset, pulses=np.asarray(spec.pulses[i]),
-                        t_start=float(spec.t_start[i]), t_end=float(spec.t_end[i]),
-                        step_size=spec.step_size, print_step=spec.print_step)
-            nv = spec.nv[i] if spec.nv else (1, 1, 1)
-            items.append({"problem": p, "H": p.H, "X": p.X, "nv": tuple(nv), "mode": MODE_2DES})
+        items = _items(H, X, alpha, alpha_t, gamma, spec)
         # the adjoint's state (trajectory or checkpoints) only when a gradient is wanted
         need = bool(want_grad)
-        b = Batch(spec.ctx, items, adjoint=spec.checkpoint if need else 0)
-        b.run()
-        b.sync()
-        outs = [b.output(i) for i in range(B)]
+        budget = _budget(spec)
+        groups = _groups(items, spec.checkpoint if need else 0, budget)
+        outs = [None] * B
+        ctx.batch, ctx.groups = None, None
+        if need and len(groups) == 1:
+            b = Batch(spec.ctx, items, adjoint=spec.checkpoint)
+            b.run()
+            b.sync()
+            outs = [b.output(i) for i in range(B)]
+            ctx.batch = b
+            spec.timing["forward_ms"] = b.elapsed_ms()
+        else:
+            # plain forward batches; with a gradient wanted, the backward re-runs `groups`
+            fwd = _groups(items, 0, budget) if need else groups
+            ms = 0.0
+            for grp in fwd:
+                b = Batch(spec.ctx, [items[i] for i in grp])
+                b.run()
+                b.sync()
+                for k, i in enumerate(grp):
+                    outs[i] = b.output(k)
+                ms += b.elapsed_ms()
+                b.close()
+                if spec.progress:
+                    spec.progress(f"forward batch of {len(grp)} propagations done")
+            spec.timing["forward_ms"] = ms
+            if need:
+                ctx.groups, ctx.items = groups, items
         lens = [len(o) for o in outs]
         P = np.zeros((B, max(lens) if lens else 0), dtype=np.complex128)
         for i, o in enumerate(outs):
             P[i, : len(o)] = o
-        ctx.batch = b if need else None
         ctx.lens = lens
         ctx.spec = spec
-        spec.timing["forward_ms"] = b.elapsed_ms()
-        spec.timing["forward_adjoint"] = need
+        ctx.need = need
+        spec.timing["forward_adjoint"] = need and ctx.batch is not None
+        spec.timing["sub_batches"] = len(groups) if need else 0
         return torch.from_numpy(P)
 
     @staticmethod
     def backward(ctx, gP):
-        b = ctx.batch
-        if b is None:
+        if not ctx.need:
             raise RuntimeError("propagate: the forward ran without gradients (grad mode off or no input "
                                "requiring grad), so there is no adjoint state")
         g = gP.detach().cpu().numpy().astype(np.complex128)
-        b.backward([g[i, :n] for i, n in enumerate(ctx.lens)])
-        ctx.spec.timing["backward_ms"] = b.elapsed_ms()
-        grads = [b.grad(i) for i in range(len(ctx.lens))]
-        ctx.batch = None   # release the device state (the trajectory) with the graph
+        b = ctx.batch
+        if b is not None:
+            b.backward([g[i, :n] for i, n in enumerate(ctx.lens)])
+            ctx.spec.timing["backward_ms"] = b.elapsed_ms()
+            ctx.spec.timing["recompute_forward_ms"] = 0.0
+            grads = [b.grad(i) for i in range(len(ctx.lens))]
+            ctx.batch = None   # release the device state (the trajectory) with the graph
+        else:
+            # sub-batches: each one's forward again, with checkpoints, then its backward
+            grads = [None] * len(ctx.lens)
+            fms = bms = 0.0
+            for grp in ctx.groups:
+                sb = Batch(ctx.spec.ctx, [ctx.items[i] for i in grp], adjoint=ctx.spec.checkpoint)
+                sb.run()
+                sb.sync()
+                fms += sb.elapsed_ms()
+                sb.backward([g[i, : ctx.lens[i]] for i in grp])
+                bms += sb.elapsed_ms()
+                for k, i in enumerate(grp):
+                    grads[i] = sb.grad(k)
+                sb.close()
+                if ctx.spec.progress:
+                    ctx.spec.progress(f"gradient sub-batch of {len(grp)} propagations done")
+            ctx.spec.timing["recompute_forward_ms"] = fms
+            ctx.spec.timing["backward_ms"] = fms + bms
+            ctx.groups = ctx.items = None
 
         def stack(name):
             return torch.from_numpy(np.stack([gr[name] for gr in grads]))
 
         return stack("H"), stack("X"), stack("alpha"), stack("alpha_t"), stack("gamma"), None, None
+
+
+def _budget(spec: Spec) -> int:
+    if spec.max_batch_bytes is not None:
+        return int(spec.max_batch_bytes)
+    free, _ = spec.ctx.mem_info()
+    return int(0.85 * free)
+
+
+def _groups(items, checkpoint: int, budget: int):
+    """Consecutive instances packed into batches of at most `budget` device bytes
+    (eh_batch_plan of each instance; with checkpoint > 0 including the adjoint's
+    checkpoint + replay state)."""
+    groups, cur, used = [], [], 0
+    for i, it in enumerate(items):
+        pl = batch_plan([it], checkpoint=max(1, checkpoint))
+        need = pl["forward_bytes"] + (pl["adjoint_replay_bytes"] if checkpoint > 0 else 0)
+        if need > budget:
+            raise EtoHeomError(-6, f"propagate: instance {i} needs {need} device bytes, more than the "
+                                   f"{budget} available to one batch")
+        if cur and used + need > budget:
+            groups.append(cur)
+            cur, used = [], 0
+        cur.append(i)
+        used += need
+    if cur:
+        groups.append(cur)
+    return groups
 
 
 def propagate(H, X, alpha, alpha_t, gamma, spec: Spec):

@@ -161,3 +161,63 @@ def grid_signal(params, spec, propagate=None):
 
 def grid_loss(S, target):
     return torch.sum(torch.abs(S - target) ** 2) / torch.sum(torch.abs(target) ** 2)
+
+
+# ---------------------------------------------------------------- the 10-site chain grid
+
+def chain_params(n_sites=10, seed=10, perturb=None):
+    """Site energies eps (n, cm^-1) and nearest-neighbour couplings J (n - 1, cm^-1) of
+    the chain workload as leaf tensors (the dipoles, bath and polarization stay fixed):
+    eps ~ N(12500, 150) from `seed`, J = 80 cm^-1.  perturb: a seed for +/-50 cm^-1
+    shifts of the energies and +/-10 % of the couplings (a target system)."""
+    rng = np.random.default_rng(seed)
+    eps = rng.normal(12500.0, 150.0, n_sites)
+    J = np.full(n_sites - 1, 80.0)
+    if perturb is not None:
+        r = np.random.default_rng(perturb)
+        eps = eps + r.uniform(-50, 50, n_sites)
+        J = J * (1 + r.uniform(-0.1, 0.1, n_sites - 1))
+    return {"eps": torch.tensor(eps, dtype=torch.float64, requires_grad=True),
+            "J": torch.tensor(J, dtype=torch.float64, requires_grad=True)}
+
+
+def _chain_system(params, seed=10):
+    from .configs import OHMIC_35_100_77 as F
+    n = params["eps"].shape[0]
+    rng = np.random.default_rng(seed)
+    rng.normal(12500.0, 150.0, n)   # the energies' draws, as configs.chain makes them
+    mu = rng.normal(size=(n, 3))
+    mu /= np.linalg.norm(mu, axis=1, keepdims=True)
+    Jm = torch.zeros((n, n), dtype=torch.float64)
+    for i in range(n - 1):
+        Jm[i, i + 1] = params["J"][i]
+        Jm[i + 1, i] = params["J"][i]
+    polar = np.array([[0.0] * 4, [0.0] * 4, [1.0] * 4])
+    H, X = AG.build_2des_system(params["eps"], Jm, torch.tensor(mu), F["lambda_line"], polar)
+    t = lambda v: torch.tensor([v], dtype=torch.float64)   # noqa: E731
+    al, alt, ga = AG.eto_modes(t(F["a"]), t(F["b"]), t(F["g"]), t(F["w"]))
+    return H, X, al.repeat(n, 1), alt.repeat(n, 1), ga.repeat(n, 1)
+
+
+def chain_grid_spec(ctx, t1s=(0.0, 10.0, 20.0, 30.0), t2s=(0.0, 100.0, 200.0, 400.0), nsteps=3000, L=3, dt=0.5,
+                    checkpoint=32, E0_cm=1e-4, taup=15.0, w0_cm=12050.0, t0=40.0, max_batch_bytes=None):
+    """Spec of propagations of the chain's (t1, t2) grid: one p111 propagation per point,
+    `nsteps` steps from t = 0, readouts every dt after the third pulse."""
+    pulses, nv = [], []
+    for t2 in t2s:
+        for t1 in t1s:
+            taus = (t0, t0 + t1, t0 + t1 + t2)
+            pulses.append([[E0_cm / HBAR, tau, taup, w0_cm / HBAR] for tau in taus])
+            nv.append((1, 1, 1))
+    B = len(pulses)
+    return AG.Spec(ctx=ctx, L=L, pulses=np.array(pulses), t_start=np.zeros(B),
+                   t_end=np.full(B, dt * (nsteps - 1)), step_size=dt, print_step=dt, nv=nv, checkpoint=checkpoint,
+                   max_batch_bytes=max_batch_bytes)
+
+
+def chain_grid_signal(params, spec, propagate=None):
+    """P[b, t] of the chain grid's propagations (complex tensor)."""
+    H, X, al, alt, ga = _chain_system(params)
+    B = len(spec.nv)
+    return (propagate or AG.propagate)(H.expand(B, -1, -1), X.expand(B, -1, -1, -1), al.expand(B, -1, -1),
+                                       alt.expand(B, -1, -1), ga.expand(B, -1, -1), spec)

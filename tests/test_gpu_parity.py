@@ -141,6 +141,32 @@ def test_chain_n56_mfma64_matches_oracle(ctx):
     assert rel(rho, rr) < TOL
 
 
+def test_chain_n56_every_launch_candidate_matches_oracle(ctx):
+    """N = 56 on every launch candidate of its N class (the MFMA kernels with and without
+    column copies, the generic one): 24 pulse-window steps with complex V, then 46 with
+    V = H, a batch of two instances."""
+    ps = []
+    for t1 in (0.0, 2.0):
+        p = configs.chain(L=1, nsteps=70)
+        p.pulses[:, 1] = [3.0, 6.0 + t1, 0.0]
+        p.pulses[:, 2] = 3.0
+        p.pulses[:, 0] *= 1e5
+        ps.append(p)
+    refs = [O.twod(oproblem(p), nv=(1, 1, 1), want_rho=True) for p in ps]
+    b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)} for p in ps])
+    cands = b.candidates()
+    assert len(cands) >= 3
+    for g, c, kern, _ in cands:
+        b.select(g, c)
+        b.run()
+        b.sync()
+        for i, (Pr, rr) in enumerate(refs):
+            assert_driven(rr)
+            assert rel(b.output(i), Pr) < TOL, (kern, i)
+            assert rel(b.rho(i), rr) < TOL, (kern, i)
+    b.close()
+
+
 def test_cli_2des_prints_reference_format(ctx, tmp_path):
     import subprocess
     txt = open("tests/golden/cpu_dimer_ohmic.key-tmpl").read()
