@@ -393,6 +393,20 @@ int eh_2des_solve(eh_ctx* ctx, const eh_system* sys, const double* H, const doub
     return solve_one(ctx, in, P_out, cap, n_points, rho_out);
 }
 
+int eh_2des_solve_noalloc(eh_ctx* ctx, const eh_system* sys, const double* H, const double* X4, int nv1, int nv2,
+                          int nv3, double* P_out, int64_t cap, int64_t* n_points, double* rho_out) {
+    eh_instance in;
+    std::memset(&in, 0, sizeof(in));
+    in.sys = sys;
+    in.H = H;
+    in.X4 = X4;
+    in.nv[0] = nv1;
+    in.nv[1] = nv2;
+    in.nv[2] = nv3;
+    in.mode = EH_MODE_2DES;
+    return solve_2des_noalloc(ctx, in, P_out, cap, n_points, rho_out);
+}
+
 int eh_dynamics_solve(eh_ctx* ctx, const eh_system* sys, const double* H, const int* sites, int n_sites,
                       double* pop_out, int64_t cap, int64_t* n_points, double* rho_out) {
     eh_instance in;

@@ -882,7 +882,7 @@ int ctx_init(int device, eh_ctx* ctx) {
     std::snprintf(ctx->name, sizeof(ctx->name), "%s (%s)", prop.name, prop.gcnArchName);
     ctx->n_cu = prop.multiProcessorCount;
     ctx->lds_per_block = (int)prop.sharedMemPerBlock;
-    return EH_OK;
+    return scratch_setup(device);   // the allocation-free entry point's array on this device
 }
 
 int ctx_mem_info(eh_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes) {
@@ -993,15 +993,19 @@ PrologueArgs prologue_args(const BatchImpl* im) {
     return pa;
 }
 
+void launch_reset_raw(hipStream_t s, double2* y, long long total, double2* yc, long long ctotal, int* ctr,
+                      const InstDev* inst, int n_inst) {
+    const long long mx = std::max(total, ctotal);
+    const int blocks = (int)std::min<long long>(std::max<long long>(1, (mx + 255) / 256), 65536);
+    hipLaunchKernelGGL(reset_state, dim3(blocks), dim3(256), 0, s, y, total, yc, ctotal, ctr);
+    hipLaunchKernelGGL(set_ground, dim3((n_inst + 255) / 256), dim3(256), 0, s, y, yc, inst, n_inst);
+}
+
 // every state to rho_0 = |0><0| (and its column copy), step counter to 0
 static void launch_reset(Batch* b) {
     BatchImpl* im = b->impl;
-    const long long total = std::max((long long)b->total_elems, (long long)im->Yc.n);
-    const int blocks = (int)std::min<long long>(std::max<long long>(1, (total + 255) / 256), 65536);
-    hipLaunchKernelGGL(reset_state, dim3(blocks), dim3(256), 0, im->stream, im->Y.p, (long long)b->total_elems,
-                       im->Yc.p, (long long)im->Yc.n, im->step_ctr.p);
-    hipLaunchKernelGGL(set_ground, dim3((b->n_inst + 255) / 256), dim3(256), 0, im->stream, im->Y.p, im->Yc.p,
-                       im->inst.p, b->n_inst);
+    launch_reset_raw(im->stream, im->Y.p, (long long)b->total_elems, im->Yc.p, (long long)im->Yc.n, im->step_ctr.p,
+                     im->inst.p, b->n_inst);
     PrologueArgs pa = prologue_args(im);   // V of step 0 (the fused graph's first step reads it)
     pa.readout = 0;
     hipLaunchKernelGGL(prologue, dim3(b->n_inst), dim3(256), 0, im->stream, pa);
@@ -1055,27 +1059,30 @@ static int capture_forward(Batch* b) {
     return EH_OK;
 }
 
-int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
-    if (!ctx || !in || n <= 0 || !out) return fail(EH_EINVAL, "batch_create: bad arguments");
-    HIPCHK(hipSetDevice(ctx->device));
-    std::unique_ptr<Batch> b(new Batch());
-    std::unique_ptr<BatchImpl> im(new BatchImpl());
+// The host side of batch_create: every check, table, tile list, group and launch
+// candidate of the n instances, with no device allocation or copy (the only device
+// call is hipFuncSetAttribute for the candidates whose LDS exceeds 64 KB).  Fills
+// b's sizes and per-instance info.
+int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBatch& hb) {
     b->ctx = ctx;
     b->n_inst = n;
-    b->info.resize(n);
-
-    std::vector<InstDev> hinst(n);
-    std::vector<int> t_inst, t_ado0, t_nado;
-    std::vector<int> hup, hdn, hpidx;
-    std::vector<uint8_t> hado;
-    std::vector<double> hcx, hcy;
+    b->info.assign((size_t)n, Batch::Info());
+    b->ado_updates = 0;
+    b->max_steps = 0;
+    std::vector<InstDev>& hinst = hb.inst;
+    hinst.assign((size_t)n, InstDev());
+    std::vector<int>&t_inst = hb.t_inst, &t_ado0 = hb.t_ado0, &t_nado = hb.t_nado;
+    std::vector<int>&hup = hb.up, &hdn = hb.dn, &hpidx = hb.pidx;
+    std::vector<uint8_t>& hado = hb.ado;
+    std::vector<double>&hcx = hb.cx, &hcy = hb.cy;
     // Hierarchy tables (neighbour indices, occupations, link coefficients) once per
     // distinct (d, L, alpha): they depend on nothing else, so a grid of propagations
     // with one bath shares a few small, cache-resident tables.
     std::map<std::vector<double>, int64_t> hier_off;
-    std::vector<uint8_t> hvreal;
-    std::vector<double2> hH, hXp, hfield, hR, hal, halt, hdamp;
-    std::vector<long long> hado0;
+    std::vector<uint8_t>& hvreal = hb.vreal;
+    std::vector<double2>&hH = hb.H, &hXp = hb.Xp, &hfield = hb.field, &hR = hb.R, &hal = hb.al, &halt = hb.alt,
+                         &hdamp = hb.damp;
+    std::vector<long long>& hado0 = hb.ado0;
     int64_t state_off = 0, p_off = 0, lk_off = 0, col_off = 0, vp_off = 0;
     int ado_off = 0, v_off = 0, r_off = 0, step_off = 0, al_off = 0;
     // instances by N class: 1..16 (each N its own class), 17 (16 < N <= 32), 18 (32 < N <= 64)
@@ -1209,9 +1216,14 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
     }
     b->total_ados = ado_off;
     b->total_elems = state_off;
+    hb.state_elems = state_off;
+    hb.p_elems = p_off;
+    hb.col_elems = col_off;
+    hb.vp_elems = vp_off;
     // Groups and their launch candidates.  Tiles are (instance, first ADO, ADO count)
     // triples; a group's candidates with equal ADOs per tile share one tile range.
-    std::vector<int> hginst;
+    std::vector<int>& hginst = hb.ginst;
+    hb.groups.clear();
     // XCD-aware tile order: workgroups go to the XCDs round-robin (workgroup w to XCD
     // w mod xcd), each XCD with its own L2, so a range of tiles is laid out such that
     // XCD x runs the x-th contiguous slice of the (instance, ADO) order: an instance's
@@ -1371,20 +1383,30 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
         g.tile_begin = g.cand[0].tile_begin;
         g.tile_count = g.cand[0].tile_count;
         g.shmem = g.cand[0].shmem;
-        im->groups.push_back(g);
+        hb.groups.push_back(g);
     }
+    return EH_OK;
+}
 
-    int rc;
-    std::vector<InstDev> hi = hinst;
-    if ((rc = im->inst.upload(hi)) || (rc = im->tile_inst.upload(t_inst)) || (rc = im->tile_ado0.upload(t_ado0)) ||
-        (rc = im->tile_nado.upload(t_nado)) || (rc = im->nb_up.upload(hup)) || (rc = im->nb_dn.upload(hdn)) ||
-        (rc = im->cx.upload(hcx)) || (rc = im->cy.upload(hcy)) || (rc = im->pidx.upload(hpidx)) ||
-        (rc = im->vreal.upload(hvreal)) || (rc = im->H.upload(hH)) || (rc = im->Xp.upload(hXp)) ||
-        (rc = im->field.upload(hfield)) || (rc = im->R.upload(hR)) || (rc = im->alpha.upload(hal)) ||
-        (rc = im->alpha_t.upload(halt)) || (rc = im->damp.upload(hdamp)) || (rc = im->ado0_off.upload(hado0)) ||
-        (rc = im->ginst.upload(hginst)))
+int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
+    if (!ctx || !in || n <= 0 || !out) return fail(EH_EINVAL, "batch_create: bad arguments");
+    HIPCHK(hipSetDevice(ctx->device));
+    std::unique_ptr<Batch> b(new Batch());
+    std::unique_ptr<BatchImpl> im(new BatchImpl());
+    HostBatch hb;
+    int rc = build_host_batch(ctx, n, in, b.get(), hb);
+    if (rc) return rc;
+    im->groups = hb.groups;
+    const int64_t state_off = hb.state_elems, p_off = hb.p_elems, col_off = hb.col_elems, vp_off = hb.vp_elems;
+    if ((rc = im->inst.upload(hb.inst)) || (rc = im->tile_inst.upload(hb.t_inst)) ||
+        (rc = im->tile_ado0.upload(hb.t_ado0)) || (rc = im->tile_nado.upload(hb.t_nado)) ||
+        (rc = im->nb_up.upload(hb.up)) || (rc = im->nb_dn.upload(hb.dn)) || (rc = im->cx.upload(hb.cx)) ||
+        (rc = im->cy.upload(hb.cy)) || (rc = im->pidx.upload(hb.pidx)) || (rc = im->vreal.upload(hb.vreal)) ||
+        (rc = im->H.upload(hb.H)) || (rc = im->Xp.upload(hb.Xp)) || (rc = im->field.upload(hb.field)) ||
+        (rc = im->R.upload(hb.R)) || (rc = im->alpha.upload(hb.al)) || (rc = im->alpha_t.upload(hb.alt)) ||
+        (rc = im->damp.upload(hb.damp)) || (rc = im->ado0_off.upload(hb.ado0)) || (rc = im->ginst.upload(hb.ginst)))
         return rc;
-    if ((rc = im->step_ctr.alloc(1)) || (rc = im->V.alloc(2 * hH.size())) || (rc = im->P.alloc((size_t)p_off)) ||
+    if ((rc = im->step_ctr.alloc(1)) || (rc = im->V.alloc(2 * hb.H.size())) || (rc = im->P.alloc((size_t)p_off)) ||
         (rc = im->Y.alloc((size_t)state_off)) || (rc = im->ACC.alloc((size_t)state_off)) ||
         (rc = im->S1.alloc((size_t)state_off)) || (rc = im->S2.alloc((size_t)state_off)) ||
         (rc = im->Yc.alloc((size_t)col_off)) || (rc = im->S1c.alloc((size_t)col_off)) ||
@@ -1402,16 +1424,16 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
     fz.R = im->R.p;
     fz.P = im->P.p;
     if ((rc = im->fz.upload(std::vector<FusedDev>{fz}))) return rc;
-    im->hinst = hinst;
-    im->hup = std::move(hup);
-    im->hdn = std::move(hdn);
-    im->hcx = std::move(hcx);
-    im->hcy = std::move(hcy);
-    im->hal = std::move(hal);
-    im->halt = std::move(halt);
-    im->hdamp = std::move(hdamp);
-    im->hado = std::move(hado);
-    im->htile_inst = t_inst;
+    im->hinst = std::move(hb.inst);
+    im->hup = std::move(hb.up);
+    im->hdn = std::move(hb.dn);
+    im->hcx = std::move(hb.cx);
+    im->hcy = std::move(hb.cy);
+    im->hal = std::move(hb.al);
+    im->halt = std::move(hb.alt);
+    im->hdamp = std::move(hb.damp);
+    im->hado = std::move(hb.ado);
+    im->htile_inst = std::move(hb.t_inst);
     HIPCHK(hipStreamCreateWithFlags(&im->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&im->ev0));
     HIPCHK(hipEventCreate(&im->ev1));

@@ -53,6 +53,9 @@ int batch_enable_adjoint(Batch* b, int checkpoint_every);
 int batch_backward(Batch* b, const double* dP, int64_t n_dP);
 int batch_get_grad(Batch* b, int i, double* dH, double* dX4, double* dalpha, double* dalpha_t, double* dgamma);
 int ctx_init(int device, eh_ctx* ctx);
+// one 2DES propagation over the static device scratch array (noalloc.hip)
+int solve_2des_noalloc(eh_ctx* ctx, const eh_instance& in, double* P_out, int64_t cap, int64_t* n_points,
+                       double* rho_out);
 int ctx_mem_info(eh_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
 
 }  // namespace eh

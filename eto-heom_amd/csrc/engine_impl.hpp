@@ -76,6 +76,31 @@ struct Group {
     int sel = 0;                  // the candidate the forward graph launches
 };
 
+// Host side of a batch (build_host_batch): the tables batch_create uploads, the
+// groups with their launch candidates, and the element counts of the device buffers.
+struct HostBatch {
+    std::vector<InstDev> inst;
+    std::vector<int> t_inst, t_ado0, t_nado, up, dn, pidx, ginst;
+    std::vector<uint8_t> ado, vreal;
+    std::vector<double> cx, cy;
+    std::vector<double2> H, Xp, field, R, al, alt, damp;
+    std::vector<long long> ado0;
+    std::vector<Group> groups;
+    int64_t state_elems = 0;   // complex elements of each state buffer (Y, ACC, S1, S2)
+    int64_t p_elems = 0;       // complex readouts
+    int64_t col_elems = 0;     // complex elements of each coupled-column copy
+    int64_t vp_elems = 0;      // doubles of one copy of the padded V planes
+};
+int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBatch& hb);
+// reset_state + set_ground of engine.hip on raw buffers: every state element and
+// column copy to zero, rho_0 = |0><0| per instance, step counter to 0
+void launch_reset_raw(hipStream_t s, double2* y, long long total, double2* yc, long long ctotal, int* ctr,
+                      const InstDev* inst, int n_inst);
+
+// noalloc.hip: the device address of its static scratch array on `device` (current),
+// looked up once per device (ctx_init)
+int scratch_setup(int device);
+
 struct AdjState;   // adjoint.hip
 void adj_destroy(AdjState* a);
 

@@ -38,7 +38,7 @@ EXPORTS = [
     "eh_last_error", "eh_problem_from_key", "eh_problem_destroy", "eh_problem_dims", "eh_problem_system",
     "eh_problem_get", "eh_problem_polar_default", "eh_problem_set_polar", "eh_polar_rotate",
     "eh_construct_hal", "eh_problem_samples", "eh_rng_stream", "eh_hierarchy_size", "eh_hierarchy_build",
-    "eh_ctx_create", "eh_ctx_destroy", "eh_ctx_info", "eh_2des_solve", "eh_dynamics_solve",
+    "eh_ctx_create", "eh_ctx_destroy", "eh_ctx_info", "eh_2des_solve", "eh_2des_solve_noalloc", "eh_dynamics_solve",
     "eh_2des_key_run", "eh_dynamics_key_run", "eh_batch_create", "eh_batch_destroy", "eh_batch_run",
     "eh_batch_sync", "eh_batch_elapsed_ms", "eh_batch_get_output", "eh_batch_get_rho", "eh_batch_dims",
     "eh_batch_ado_updates", "eh_batch_profile", "eh_batch_enable_adjoint", "eh_batch_backward",
@@ -102,6 +102,8 @@ def lib():
             "eh_ctx_destroy": (C.c_int, [VP]),
             "eh_ctx_info": (C.c_int, [VP, C.c_char_p, C.c_int, P(C.c_int), P(C.c_int)]),
             "eh_2des_solve": (C.c_int, [VP, P(System), VP, VP, C.c_int, C.c_int, C.c_int, VP, I64, P(I64), VP]),
+            "eh_2des_solve_noalloc": (C.c_int, [VP, P(System), VP, VP, C.c_int, C.c_int, C.c_int, VP, I64,
+                                                P(I64), VP]),
             "eh_dynamics_solve": (C.c_int, [VP, P(System), VP, VP, C.c_int, VP, I64, P(I64), VP]),
             "eh_2des_key_run": (C.c_int, [VP, C.c_char_p, VP, I64, P(I64)]),
             "eh_dynamics_key_run": (C.c_int, [VP, C.c_char_p, VP, C.c_int, VP, I64, P(I64)]),
@@ -374,6 +376,24 @@ def twoD_spectrum_solver(ctx: Context, key, H, nv1: int, nv2: int, nv3: int, X=N
     rho = np.zeros((M, s.N, s.N), dtype=np.complex128) if want_rho else None
     rc = lib().eh_2des_solve(ctx._h, C.byref(s), Hc.ctypes.data, Xc.ctypes.data, nv1, nv2, nv3,
                              P.ctypes.data, cap, C.byref(n), _ptr(rho))
+    _check(rc, allow_nan=True)
+    return (P[: n.value].copy(), rho) if want_rho else P[: n.value].copy()
+
+
+def twoD_spectrum_solver_noalloc(ctx: Context, key, H, nv1: int, nv2: int, nv3: int, X=None, want_rho=False):
+    """twoD_spectrum_solver through eh_2des_solve_noalloc: no device allocation (the
+    library's fixed scratch array; EH_ERANGE for propagations that do not fit it)."""
+    s, keep = _sys_of(key)
+    if X is None:
+        X = key.get("X") if isinstance(key, Param) else key.X
+    Hc, Xc = _c128(H), _c128(X)
+    cap = _nsteps(s) + 1
+    P = np.zeros(cap, dtype=np.complex128)
+    n = C.c_int64(0)
+    M = hierarchy_size(s.K * s.Km, s.L)
+    rho = np.zeros((M, s.N, s.N), dtype=np.complex128) if want_rho else None
+    rc = lib().eh_2des_solve_noalloc(ctx._h, C.byref(s), Hc.ctypes.data, Xc.ctypes.data, nv1, nv2, nv3,
+                                     P.ctypes.data, cap, C.byref(n), _ptr(rho))
     _check(rc, allow_nan=True)
     return (P[: n.value].copy(), rho) if want_rho else P[: n.value].copy()
 

@@ -116,6 +116,23 @@ int eh_ctx_mem_info(eh_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
 int eh_2des_solve(eh_ctx* ctx, const eh_system* sys, const double* H, const double* X4,
                   int nv1, int nv2, int nv3, double* P_out, int64_t cap, int64_t* n_points,
                   double* rho_out);
+/* ---- one propagation that allocates no device memory ------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device memory
+ * (serving loops, harnesses that forbid hipMalloc).  Same arguments, checks and
+ * results as eh_2des_solve (bit for bit: the same kernels, launched one by one),
+ * computed in one static device array of EH_SCRATCH_BYTES per GPU that the library
+ * holds from eh_ctx_create until the process ends; it calls no hipMalloc or hipFree and
+ * creates no stream (the null stream), and returns after the outputs are copied back.
+ * EH_ERANGE when the propagation's four state buffers and tables exceed the array
+ * (about 4 M N^2 complex: the dimer, trimer and tetramer hierarchies fit; the
+ * 3060-ADO N = 29 heptamer, about 170 MB, needs eh_2des_solve).  Every call on one
+ * GPU shares the array and overwrites it; a per-device lock held for the whole call
+ * makes calls from several threads of one process run one after another instead of
+ * racing on it. */
+#define EH_SCRATCH_BYTES (64LL << 20)
+int eh_2des_solve_noalloc(eh_ctx* ctx, const eh_system* sys, const double* H, const double* X4,
+                          int nv1, int nv2, int nv3, double* P_out, int64_t cap, int64_t* n_points,
+                          double* rho_out);
 /* dynamics_solver (dynamics.cpp:202-295): pop_out[s*cap + t] = Re rho_0(site_s-1, site_s-1) */
 int eh_dynamics_solve(eh_ctx* ctx, const eh_system* sys, const double* H, const int* sites,
                       int n_sites, double* pop_out, int64_t cap, int64_t* n_points,
