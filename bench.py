@@ -200,6 +200,7 @@ def bench_gradient(a):
     import etoheom_amd as E
     from etoheom_amd import spectra as SP
     ctx = E.Context(0)
+    print(f"bench: {a.workload} on {ctx.name}", file=sys.stderr, flush=True)
     if a.workload == "chain_grad":
         spec = SP.chain_grid_spec(ctx, t1s=(0.0, 10.0, 20.0, 30.0), t2s=(0.0, 100.0, 200.0, 400.0), nsteps=3000)
         mk = SP.chain_params

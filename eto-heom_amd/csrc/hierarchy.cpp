@@ -127,8 +127,9 @@ int build_tables(const Hierarchy& h, const eh_system& sys, Tables& t) {
 int build_schedule(const eh_system& sys, int mode, const int nv[3], Schedule& s) {
     if (!(sys.step_size > 0)) return fail(EH_EINVAL, "TIME step_size must be positive");
     const float nstep = (sys.t_end - sys.t_start) / sys.step_size;   // float, dynamics.cpp:344
-    const int print_step = (int)(sys.print_step / sys.step_size);     // :346
-    if (print_step <= 0) return fail(EH_EINVAL, "print_step / step_size must be >= 1");
+    const float pr = sys.print_step / sys.step_size;
+    if (!(pr >= 1.0f && pr < 2147483647.0f)) return fail(EH_EINVAL, "print_step / step_size must be in [1, 2^31)");
+    const int print_step = (int)pr;                                    // :346
     if (!(nstep < 1e8f)) return fail(EH_ERANGE, "too many time steps");
     s.nsteps = 0;
     for (int i = 0; i <= nstep; i++) s.nsteps++;

@@ -20,7 +20,14 @@ namespace eh {
 
 namespace {
 
-int nsys_2des(int n) { return n + (n - 1) * (n - 2) / 2; }
+// system dimension of a 2DES key (ground + singles + doubles), in 64 bits
+int64_t nsys_2des(int64_t n) { return n + (n - 1) * (n - 2) / 2; }
+
+// Largest sys_size a key may declare: its N x N blocks (Hamiltonian, disorder, three
+// dipole operators) are allocated as the blocks are read.
+constexpr int64_t kMaxKeyN = 2048;
+// Largest mode count of a BATH block (the hierarchy dimension K K_m is at most 255).
+constexpr int kMaxModes = 255;
 
 bool read_row(std::istream& f, std::string& line) { return static_cast<bool>(std::getline(f, line)); }
 
@@ -48,7 +55,10 @@ int parse_key(const std::string& path, int mode, Problem& k) {
             std::istringstream v(line);
             v >> k.single_size;
             if (k.single_size <= 0) return fail(EH_EPARSE, "SIZE must be positive");
-            k.sys_size = twod ? nsys_2des(k.single_size) : k.single_size;
+            const int64_t ns = twod ? nsys_2des(k.single_size) : k.single_size;
+            if (k.single_size > kMaxKeyN || ns > kMaxKeyN)
+                return fail(EH_EPARSE, "SIZE gives a system dimension above " + std::to_string(kMaxKeyN));
+            k.sys_size = (int)ns;
             k.d_map.assign(k.sys_size, {0, 0});
             if (twod) {
                 int ct = k.single_size;
@@ -61,6 +71,7 @@ int parse_key(const std::string& path, int mode, Problem& k) {
             read_row(f, line);
             std::istringstream v(line);
             v >> k.n_sample >> k.seed;
+            if (k.n_sample < 0) return fail(EH_EPARSE, "DISORDER n_sample must not be negative");
             const int N = k.sys_size;
             k.Disorder.assign((size_t)N * N, 0.0);
             for (int i = 0; i < N; i++) {
@@ -100,7 +111,7 @@ int parse_key(const std::string& path, int mode, Problem& k) {
             const int N = k.sys_size;
             for (auto& m : k.Xc) m.assign((size_t)N * N, cplx(0, 0));
             for (int q = 0; q < ndip; q++) {
-                read_row(f, line);
+                if (!read_row(f, line)) return fail(EH_EPARSE, "DIPOLE block shorter than its count");
                 std::istringstream ds(line);
                 double x = 0, y = 0, z = 0;
                 int i = 0, j = 0;
@@ -142,6 +153,7 @@ int parse_key(const std::string& path, int mode, Problem& k) {
             bs >> l;
             const int N = k.sys_size;
             const int so = twod ? 1 : 0;
+            if (k.K < 0 || k.K + so > N) return fail(EH_EPARSE, "HEOM K exceeds the number of sites");
             for (int i = 0; i < k.K; i++) k.lambda.push_back(l);
             for (int i = 0; i < k.K; i++) {
                 if (i + so >= N) return fail(EH_EPARSE, "HEOM K exceeds the number of sites");
@@ -152,9 +164,11 @@ int parse_key(const std::string& path, int mode, Problem& k) {
             std::istringstream ms(line);
             int n_modes = 0;
             ms >> n_modes;
+            if (n_modes < 0 || n_modes > kMaxModes)
+                return fail(EH_EPARSE, "BATH mode count must be in 0.." + std::to_string(kMaxModes));
             std::vector<cplx> al, alt, ga;
             for (int i = 0; i < n_modes; i++) {
-                read_row(f, line);
+                if (!read_row(f, line)) return fail(EH_EPARSE, "BATH block shorter than its mode count");
                 std::istringstream md(line);
                 double a = 0, b = 0, g = 0, w = 0;
                 md >> a >> b >> g >> w;

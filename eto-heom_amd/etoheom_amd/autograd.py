@@ -93,6 +93,8 @@ class _Propagate(torch.autograd.Function):
         ctx.batch, ctx.groups = None, None
         if need and len(groups) == 1:
             b = Batch(spec.ctx, items, adjoint=spec.checkpoint)
+            if spec.progress:
+                spec.progress(f"forward batch of {B} propagations with adjoint state: running")
             b.run()
             b.sync()
             outs = [b.output(i) for i in range(B)]
@@ -104,6 +106,8 @@ class _Propagate(torch.autograd.Function):
             ms = 0.0
             for grp in fwd:
                 b = Batch(spec.ctx, [items[i] for i in grp])
+                if spec.progress:
+                    spec.progress(f"forward batch of {len(grp)} propagations: running")
                 b.run()
                 b.sync()
                 for k, i in enumerate(grp):
@@ -134,6 +138,8 @@ class _Propagate(torch.autograd.Function):
         g = gP.detach().cpu().numpy().astype(np.complex128)
         b = ctx.batch
         if b is not None:
+            if ctx.spec.progress:
+                ctx.spec.progress(f"backward of {len(ctx.lens)} propagations: running")
             b.backward([g[i, :n] for i, n in enumerate(ctx.lens)])
             ctx.spec.timing["backward_ms"] = b.elapsed_ms()
             ctx.spec.timing["recompute_forward_ms"] = 0.0
@@ -145,9 +151,13 @@ class _Propagate(torch.autograd.Function):
             fms = bms = 0.0
             for grp in ctx.groups:
                 sb = Batch(ctx.spec.ctx, [ctx.items[i] for i in grp], adjoint=ctx.spec.checkpoint)
+                if ctx.spec.progress:
+                    ctx.spec.progress(f"gradient sub-batch of {len(grp)} propagations: forward with checkpoints")
                 sb.run()
                 sb.sync()
                 fms += sb.elapsed_ms()
+                if ctx.spec.progress:
+                    ctx.spec.progress(f"gradient sub-batch of {len(grp)} propagations: backward")
                 sb.backward([g[i, : ctx.lens[i]] for i in grp])
                 bms += sb.elapsed_ms()
                 for k, i in enumerate(grp):
