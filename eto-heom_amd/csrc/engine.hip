@@ -536,6 +536,119 @@ __global__ __launch_bounds__(512, 2) void stage_mfma64(StageArgs A) {
     if (A.fused && a0 == 0) fused_tail(A, I, step, threadIdx.x, blockDim.x);
 }
 
+// Coupling term of output element (row, col) of the ADO whose table row starts at lrow,
+// gathered straight from the stage input: the same element of the up / down neighbours
+// of the bath whose site is row (weights i x, i alpha y) and of the bath whose site is
+// col (-i x, -i alpha~ y), summed in gather_couplings' order (modes k, up then down;
+// the row term, then the column term).
+__device__ __forceinline__ double2 coupling_direct(const StageArgs& A, const InstDev& I, long long lrow, int row,
+                                                   int col) {
+    const int N = I.N, K = I.K, Km = I.Km;
+    const long long e = (long long)row * N + col;
+    const double2* rb = A.rin + I.state_off;
+    double2 c = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int rc = 0; rc < 2; ++rc) {
+        const int j = (rc == 0 ? row : col) - I.so;
+        if (j < 0 || j >= K) continue;
+        const double sg = rc == 0 ? A.gsign : -A.gsign;
+        double2 acc = make_double2(0.0, 0.0);
+        for (int k = 0; k < Km; ++k) {
+            const long long q = lrow + j + k * K;
+            const int mu = A.nb_up[q], md = A.nb_dn[q];
+            const double2 xu = mu >= 0 ? rb[(long long)mu * N * N + e] : make_double2(0.0, 0.0);
+            const double2 xd = md >= 0 ? rb[(long long)md * N * N + e] : make_double2(0.0, 0.0);
+            const double x = A.cx[q], y = A.cy[q];
+            const double2 al = rc == 0 ? A.alpha[I.al_off + j * Km + k] : A.alpha_t[I.al_off + j * Km + k];
+            acc = cmad(acc, make_double2(0.0, sg * x), xu);
+            acc = cmad(acc, make_double2(-sg * al.y * y, sg * al.x * y), xd);
+        }
+        c.x += acc.x;
+        c.y += acc.y;
+    }
+    return c;
+}
+
+// N in (64, 128]: one ADO per workgroup of 256 threads and nothing in LDS (rho_n alone is
+// 256 KB at N = 128).  Each wave walks its 16 x 16 output tiles; the MFMA operands (V
+// and rho_n, zero beyond N) are loaded straight from global memory through the caches,
+// and each output element's coupling term is gathered from the neighbours' same element
+// (coupling_direct).
+__global__ __launch_bounds__(256) void stage_mfma_wide(StageArgs A) {
+    const int tile = blockIdx.x;
+    const InstDev I = A.inst[A.tile_inst[tile]];   // register copy, read before any store
+    const int step = *A.step_ctr + A.step_off;
+    if (step >= I.nsteps) return;
+    at_step(A, step);
+    const int N = I.N, NN = N * N;
+    const int a0 = A.tile_ado0[tile];
+    const long long base = I.state_off + (long long)a0 * NN;
+    const double2* R = A.rin + base;
+    const double2* V = A.V + I.v_off;
+    const bool vreal = A.vreal[I.step_off + step] != 0;
+    const double2 g = A.damp[I.ado_off + a0];
+    const long long lrow = I.lk_off + (long long)a0 * I.d;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int T = (N + 15) >> 4, kmax = (N + 3) & ~3;
+    const double2 z = make_double2(0.0, 0.0);
+    for (int tt = wave; tt < T * T; tt += 4) {
+        const int ti = tt / T, tj = tt - ti * T;
+        const int ar = ti * 16 + li, bc = tj * 16 + li;   // A-operand row, B-operand column
+        dbl4 cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+        if (vreal) {
+#pragma unroll 2
+            for (int k0 = 0; k0 < kmax; k0 += 4) {
+                const int kk = k0 + lk;
+                const bool ain = ar < N && kk < N, bin = kk < N && bc < N;
+                const int ia = ain ? ar * N + kk : 0, ib = bin ? kk * N + bc : 0;   // (ar, kk), (kk, bc)
+                const double va0 = V[ia].x, vb0 = V[ib].x;
+                const double2 ra0 = R[ia], rb0 = R[ib];
+                const double va = ain ? va0 : 0.0, vb = bin ? vb0 : 0.0;
+                const double2 ra = ain ? ra0 : z, rb = bin ? rb0 : z;
+                cr = mfma4(va, rb.x, cr);
+                ci = mfma4(va, rb.y, ci);
+                cr = mfma4(ra.x, -vb, cr);
+                ci = mfma4(ra.y, -vb, ci);
+            }
+        } else {
+#pragma unroll 2
+            for (int k0 = 0; k0 < kmax; k0 += 4) {
+                const int kk = k0 + lk;
+                const bool ain = ar < N && kk < N, bin = kk < N && bc < N;
+                const int ia = ain ? ar * N + kk : 0, ib = bin ? kk * N + bc : 0;
+                const double2 va0 = V[ia], vb0 = V[ib], ra0 = R[ia], rb0 = R[ib];
+                const double2 va = ain ? va0 : z, vb = bin ? vb0 : z;
+                const double2 ra = ain ? ra0 : z, rb = bin ? rb0 : z;
+                cr = mfma4(va.x, rb.x, cr);
+                ci = mfma4(va.x, rb.y, ci);
+                cr = mfma4(-va.y, rb.y, cr);
+                ci = mfma4(va.y, rb.x, ci);
+                cr = mfma4(ra.x, -vb.x, cr);
+                ci = mfma4(ra.y, -vb.x, ci);
+                cr = mfma4(ra.y, vb.y, cr);
+                ci = mfma4(ra.x, -vb.y, ci);
+            }
+        }
+        const int col = tj * 16 + li;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = ti * 16 + lk + 4 * r;
+            if (row < N && col < N) {
+                const size_t gi = (size_t)(base + row * N + col);
+                double dr = ci[r], di = -cr[r];   // -i * C
+                const double2 x = A.rin[gi];
+                dr = fma(g.x, x.x, fma(-g.y, x.y, dr));
+                di = fma(g.x, x.y, fma(g.y, x.x, di));
+                const double2 cpl = coupling_direct(A, I, lrow, row, col);
+                const double2 w = rk_operand(A, gi, x);
+                rk_store(A, I, gi, x, w, w, dr + cpl.x, di + cpl.y, a0, row, col);
+            }
+        }
+    }
+    if (A.fused && a0 == 0) fused_tail(A, I, step, threadIdx.x, blockDim.x);
+}
+
 // rho_0 = |0><0| for every instance, all other ADOs zero; step counter to 0
 // Persistent, software-pipelined variant of stage_mfma for N <= 32 with at most
 // KM modes per bath and 2 K N <= 512: each workgroup walks the tiles
@@ -926,6 +1039,7 @@ int launch_cfg(const Group& g, const LaunchCfg& c, const StageArgs& base, hipStr
         case -34: hipLaunchKernelGGL((stage_mfma<32, true, true>), grid, block, c.shmem, s, a); break;
         case -64: hipLaunchKernelGGL((stage_mfma<64, false>), grid, block, c.shmem, s, a); break;
         case -65: hipLaunchKernelGGL(stage_mfma64, grid, dim3(512), c.shmem, s, a); break;
+        case -128: hipLaunchKernelGGL(stage_mfma_wide, grid, block, 0, s, a); break;
         case -33:
             // the RK coefficients of this stage are arguments (one time step per group)
             hipLaunchKernelGGL(stage_mfma_pipe<2>, dim3(std::min(c.tile_count, g.persist)), block, c.shmem, s, a,
@@ -1086,7 +1200,9 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
     int64_t state_off = 0, p_off = 0, lk_off = 0, col_off = 0, vp_off = 0;
     int ado_off = 0, v_off = 0, r_off = 0, step_off = 0, al_off = 0;
     // instances by N class: 1..16 (each N its own class), 17 (16 < N <= 32), 18 (32 < N <= 64)
-    std::vector<std::vector<int>> by_class(19);
+    // instances by N class: 1..16 (each N its own class), 17 (16 < N <= 32), 18 (32 < N <= 64),
+    // 19 (64 < N <= 128)
+    std::vector<std::vector<int>> by_class(20);
     auto env_on = [](const char* v) { const char* e = std::getenv(v); return e && e[0] == '1'; };
     const bool use_mfma = !env_on("ETOHEOM_NO_MFMA");
     const bool use_pipe = env_on("ETOHEOM_PIPE");   // opt-in default: slower than stage_mfma<32> as compiled
@@ -1186,7 +1302,7 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
         d.col_off = col_off;
         col_off += (int64_t)M * sys.K * N;
         d.vp_off = -1;
-        if (N > 32) {
+        if (N > 32 && N <= 64) {
             d.vp_off = vp_off;
             vp_off += 2 * 64 * 64;
         }
@@ -1205,7 +1321,7 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
         inf.p_off = p_off;
         b->ado_updates += (int64_t)M * s.nsteps;
         b->max_steps = std::max(b->max_steps, s.nsteps);
-        by_class[(size_t)(N <= 16 ? N : (N <= 32 ? 17 : 18))].push_back(i);
+        by_class[(size_t)(N <= 16 ? N : (N <= 32 ? 17 : (N <= 64 ? 18 : 19)))].push_back(i);
         state_off += (int64_t)M * NN;
         p_off += (int64_t)nread * s.npoints;
         ado_off += M;
@@ -1253,7 +1369,7 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
             t_nado[(size_t)(begin + p)] = qn[(size_t)src];
         }
     };
-    for (int cls = 1; cls <= 18; cls++) {
+    for (int cls = 1; cls <= 19; cls++) {
         const auto& members = by_class[(size_t)cls];
         if (members.empty()) continue;
         Group g;
@@ -1333,6 +1449,8 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
                 while (mn * mn < maxnn) mn++;
                 c.shmem = (size_t)mn * (mn + 1) * sizeof(double2) + cpb;
             }
+            else if (nt == -128)
+                c.shmem = 0;   // stage_mfma_wide: nothing in LDS
             else if (nt > 0)
                 c.shmem = (size_t)G * cpb;   // the coupling rows/columns of G ADOs (the rest is static)
             else
@@ -1361,6 +1479,9 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
             add(0, 0, true, stream);
             if (use_mfma) add(-32, 0, false, stream);
             if (use_mfma && pipe_ok) add(-33, 0, false, stream);
+        } else if (cls == 19) {   // 64 < N <= 128: operands from global memory; LDS-staged VALU where it fits
+            if (use_mfma) add(-128, 0, false, stream);
+            add(0, 0, false, stream);
         } else {   // 32 < N <= 64: stage_mfma64 gathers columns directly (measured faster at N = 56)
             if (use_mfma) add(-65, 0, false, stream);
             add(0, 0, true, stream);
@@ -1705,7 +1826,8 @@ std::string batch_tuning_report(const Batch* b) {
                              : x.nt == -64 ? "stage_mfma64_v1"
                              : x.nt == -65 ? "stage_mfma64"
                              : x.nt == -33 ? "stage_mfma_pipe"
-                                           : "stage_generic";
+                             : x.nt == -128 ? "stage_mfma_wide"
+                                            : "stage_generic";
             char line[256];
             std::snprintf(line, sizeof(line),
                           "group %zu cand %zu maxN^2=%d instances=%d %s tile=%d tiles=%d cols=%d stream=%d late=%d ms_per_step=%.6f%s\n",

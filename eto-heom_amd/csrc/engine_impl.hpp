@@ -53,7 +53,7 @@ struct DevBuf {
 // One way of launching a group's stage: kernel variant and tiling.
 struct LaunchCfg {
     int nt;                   // 1..16: stage_small<nt>; -32 / -64: stage_mfma; -65: stage_mfma64;
-                              // -33: stage_mfma_pipe; 0: stage_generic
+                              // -33: stage_mfma_pipe; -128: stage_mfma_wide; 0: stage_generic
     int ge;                   // stage_small: state elements per tile (256, 512 or 1024)
     bool cols;                // keep coupled-column copies (contiguous column gathers)
     bool stream;              // non-temporal state stores (StageArgs::stream)
@@ -62,7 +62,8 @@ struct LaunchCfg {
     size_t shmem;
 };
 
-// The instances of one N class (N <= 16 each its own class; 16 < N <= 32; 32 < N <= 64).
+// The instances of one N class (N <= 16 each its own class; 16 < N <= 32; 32 < N <= 64;
+// 64 < N <= 128).
 struct Group {
     int nt;                   // base configuration (cand[0]); the adjoint kernels use its tiling
     int tile_begin, tile_count;

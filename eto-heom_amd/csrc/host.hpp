@@ -91,7 +91,7 @@ void rng_stream(unsigned long seed, int count, uint32_t* out);
 int draw_samples(const Problem& k, int n_sample, std::vector<cplx>& H, std::vector<cplx>& X);
 
 // Largest system dimension the stage kernels take.
-constexpr int kMaxN = 64;
+constexpr int kMaxN = 128;
 
 // One instance as checked on the host (plan.cpp): sizes and step counts.
 struct InstPlan {

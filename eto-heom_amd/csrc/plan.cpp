@@ -67,13 +67,13 @@ int plan_batch(int n, const eh_instance* inst, int C, BatchPlan& out) {
         // H, two V copies, X'_q, readout operators, field and step tables, readouts
         fixed += z * (NN * (6.0 + p.nread)) + (z * 3.0 + 5.0) * p.nsteps + z * (double)p.nread * p.npoints;
         vh_nn += NN;
-        if (p.N > 32) n_big++;
+        if (p.N > 32 && p.N <= 64) n_big++;
         std::vector<double> key = {(double)p.d, (double)sys.L, (double)sys.K, (double)sys.Km};
         if (p.d) key.insert(key.end(), sys.alpha, sys.alpha + 2 * p.d);
         if (tables.emplace(key, 1).second) fixed += (double)p.M * p.d * (4 + 4 + 8 + 8 + 1);
         fixed += z * p.M;   // damping
     }
-    // zero-padded 64x64 real/imaginary V planes, two copies, per N > 32 instance
+    // zero-padded 64x64 real/imaginary V planes, two copies, per 32 < N <= 64 instance
     fixed += 2.0 * 2.0 * 64 * 64 * 8 * (double)n_big;
     const double T = (double)out.total_elems, Tc = (double)out.col_elems;
     // Y, ACC, S1, S2 and the three coupled-column copies

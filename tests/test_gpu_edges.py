@@ -10,6 +10,8 @@ normwise relative, FP64):
 * population-dynamics instances with different site lists beside a 2DES instance in one
   batch (more output sites than N is an error, as in the oracle).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -139,3 +141,50 @@ def test_mixed_modes_and_site_lists_in_one_batch(ctx):
     assert rel(b.output(1), Pr) < TOL
     with pytest.raises(Exception):
         E.Batch(ctx, [{"problem": d2, "H": H, "mode": E.MODE_DYNAMICS, "sites": [2, 1, 2]}])
+
+
+def test_n67_eleven_site_aggregate_every_candidate(ctx):
+    """N = 67 (11 sites: ground + 11 singles + 55 doubles, the first size above 64),
+    L = 1 (23 ADOs), 40 steps through three strong pulses: the wide MFMA kernel (operands
+    and couplings from global memory) and the LDS-staged VALU kernel, each against the
+    oracle."""
+    p = strong(configs.aggregate(11, 1, nsteps=40))
+    assert p.N == 67
+    b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)}])
+    cands = b.candidates()
+    kinds = {k for _, _, k, _ in cands}
+    assert "stage_generic" in kinds
+    if os.environ.get("ETOHEOM_NO_MFMA") != "1":
+        assert "stage_mfma_wide" in kinds
+    Pr, rr = O.twod(oproblem(p), nv=(1, 1, 1), want_rho=True)
+    assert np.max(np.abs(rr[1:])) > 1e-12 * np.max(np.abs(rr[0]))
+    for g, c, kern, _ in cands:
+        print(f"N=67 candidate {c}: {kern}", flush=True)
+        b.select(g, c)
+        b.run()
+        b.sync()
+        assert len(b.output(0)) == len(Pr) > 0
+        assert rel(b.output(0), Pr) < TOL, kern
+        assert rel(b.rho(0), rr) < TOL, kern
+    b.close()
+
+
+def test_n128_random_hermitian_system(ctx):
+    """N = 128, the largest system: random Hermitian H, three baths, L = 1, 20 steps."""
+    base = configs.chain(L=1, nsteps=20)
+    rng = np.random.default_rng(128)
+    N = 128
+    A = rng.normal(size=(N, N)) * 0.02
+    H = np.diag(rng.uniform(2.2, 2.45, N)) + (A + A.T) / 2
+    H[0, :] = 0.0
+    H[:, 0] = 0.0
+    X = np.zeros((4, N, N), complex)
+    for q in range(4):
+        v = rng.normal(size=N - 1)
+        X[q, 0, 1:] = v
+        X[q, 1:, 0] = v
+    p = strong(like(base, H=H.astype(complex), X=X, alpha=base.alpha[:3], alpha_t=base.alpha_t[:3],
+                    gamma=base.gamma[:3]))
+    assert p.N == 128
+    check(ctx, p)
+

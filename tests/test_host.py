@@ -274,3 +274,21 @@ def test_batch_plan_bytes():
     bad["X"] = None
     with pytest.raises(E.EtoHeomError):
         E.batch_plan([bad])
+
+
+def test_batch_plan_system_size_range():
+    """N up to 128 is accepted (the N > 64 kernels keep nothing of rho in LDS); 129 is
+    refused on the host with EH_ESHAPE."""
+    def item(N):
+        al = np.full((3, 2), 3.3e-5 - 1.5e-5j)
+        p = E.Problem(H=np.eye(N, dtype=complex), alpha=al, alpha_t=al.conj(), gamma=np.full((3, 2), 0.023 + 0j),
+                      L=1, X=np.zeros((4, N, N), complex), site_offset=1,
+                      pulses=np.array([[1e-4 / HB, 40.0, 15.0, 2.27]] * 3), t_start=0.0, t_end=5.0,
+                      step_size=0.5, print_step=0.5)
+        return {"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)}
+    for N in (65, 67, 128):
+        pl = E.batch_plan([item(N)])
+        assert pl["total_elems"] == 7 * N * N
+    with pytest.raises(E.EtoHeomError) as ei:
+        E.batch_plan([item(129)])
+    assert ei.value.code == -3 and "1..128" in str(ei.value)
