@@ -166,6 +166,43 @@ __device__ void reduce_bath(const AdjArgs& J, const InstDev& I, int n0, int na, 
     }
 }
 
+// reduce_bath for one ADO whose kb sits in LDS at row stride LD (adj_mfma64)
+__device__ void reduce_bath_ld(const AdjArgs& J, const InstDev& I, int n0, const double2* sKb, int LD, double fs,
+                               double* sU, int tid, int nthr) {
+    const int N = I.N, NN = N * N, K = I.K, Km = I.Km, d = I.d;
+    const int per = 2 * K * N;
+    for (int e = tid; e < per; e += nthr) {
+        const int j = e / (2 * N), rem = e - j * 2 * N;
+        const int rc = rem / N, pos = rem - rc * N;
+        const int s = j + I.so;
+        const int ab = rc == 0 ? s * N + pos : pos * N + s;
+        double2 kb = sKb[rc == 0 ? s * LD + pos : pos * LD + s];
+        kb.x *= fs;
+        kb.y *= fs;
+        const long long row0 = I.lk_off + (long long)n0 * d;
+        for (int k = 0; k < Km; ++k) {
+            const int p = j + k * K;
+            const long long q = row0 + p;
+            const int mu = J.fnb_up[q], md = J.fnb_dn[q];
+            if (mu >= 0) {
+                const double2 yu = J.Yf[I.state_off + (long long)mu * NN + ab];
+                const double2 t = cconj_mul(kb, yu);
+                const double x = J.fcx[q] * (rc == 0 ? 1.0 : -1.0);
+                atomicAdd(&sU[2 * p], -x * t.y);
+                atomicAdd(&sU[2 * p + 1], x * t.x);
+            }
+            if (md >= 0) {
+                const double2 yd = J.Yf[I.state_off + (long long)md * NN + ab];
+                const double2 t = cconj_mul(kb, yd);
+                const double y = J.fcy[q] * (rc == 0 ? 1.0 : -1.0);
+                double* dst = sU + 2 * d * (rc == 0 ? 1 : 2);
+                atomicAdd(&dst[2 * p], -y * t.y);
+                atomicAdd(&dst[2 * p + 1], y * t.x);
+            }
+        }
+    }
+}
+
 // The coupling-parameter reductions with one thread per (ADO, bath j, row/column,
 // mode k): it sums conj(kb) rho over the N positions of row / column s = j + so
 // of the tile's ADO (kb from LDS) against the forward stage input of the up and
@@ -547,6 +584,203 @@ __global__ __launch_bounds__(256) void adj_generic(AdjArgs J) {
     }
 }
 
+// 32 < N <= 64: one ADO per workgroup of 512 threads (8 waves, each two 16 x 16 tiles of
+// one tile row, as stage_mfma64), two workgroups per CU.  LDS holds the adjoint stage
+// input l (N x (N+1) complex), the coupling rows / columns and the bath sums; the forward
+// stage input Y is read through the caches.  On FP64 MFMA:
+//   C' = V^H l - l V^H                      (the stage update u = i C' + conj(damp) l + couplings)
+//   Gt = Y K^H - K^H Y,  K = fs l           (dL/dV, added into the instance's accumulator copy)
+// with the element-wise terms, the readout fold, q_n and the bath reductions as adj_generic.
+__global__ __launch_bounds__(512, 2) void adj_mfma64(AdjArgs J) {
+    const StageArgs& A = J.S;
+    extern __shared__ double2 smem[];
+    const int tile = blockIdx.x;
+    const int ii = A.tile_inst[tile];
+    const InstDev& I = A.inst[ii];
+    const int step = *A.step_ctr + A.step_off;
+    if (step < 0 || step >= I.nsteps) return;
+    J.Yf += (long long)step * J.traj;
+    const int N = I.N, NN = N * N, d = I.d, LD = N + 1;
+    double2* sR = smem;                                               // l, N x (N+1)
+    double2* sC = smem + N * LD;                                      // 2 K N coupling rows/columns
+    double* sU = reinterpret_cast<double*>(sC + 2 * I.K * N);         // U, A, B (6 d doubles)
+    __shared__ double2 red[8];
+    const int a0 = A.tile_ado0[tile];
+    const int st = A.stage;
+    const long long base = I.state_off + (long long)a0 * NN;
+    const double2* Yf = J.Yf + base;
+    for (int e = threadIdx.x; e < NN; e += 512) {
+        const int r = e / N, c = e - r * N;
+        sR[r * LD + c] = A.rin[base + e];
+    }
+    for (int e = threadIdx.x; e < 6 * d; e += 512) sU[e] = 0.0;
+    gather_couplings(A, I, a0, 1, sC, threadIdx.x, 512);
+    __syncthreads();
+    const double2* V = A.V + I.v_off;   // V^H of this step
+    const bool vreal = A.vreal[I.step_off + step] != 0;
+    const double fs = st == 0 ? I.h / 6.0 : 1.0;
+    const double2 g = A.damp[I.ado_off + a0];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int ti = wave >> 1, tj0 = (wave & 1) * 2;
+    const int ar = ti * 16 + li;
+    const int kmax = (N + 3) & ~3;
+    dbl4 cr[2], ci[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        cr[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+        ci[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+    }
+    // C' = V^H l - l V^H
+    if (vreal) {
+#pragma unroll 2
+        for (int k0 = 0; k0 < kmax; k0 += 4) {
+            const int kk = k0 + lk;
+            const double va = gl_at(V, ar, kk, N).x;
+            const double2 ra = lds_at(sR, ar, kk, N, LD);
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int bc = (tj0 + t) * 16 + li;
+                const double vb = gl_at(V, kk, bc, N).x;
+                const double2 rb = lds_at(sR, kk, bc, N, LD);
+                cr[t] = mfma4(va, rb.x, cr[t]);
+                ci[t] = mfma4(va, rb.y, ci[t]);
+                cr[t] = mfma4(ra.x, -vb, cr[t]);
+                ci[t] = mfma4(ra.y, -vb, ci[t]);
+            }
+        }
+    } else {
+#pragma unroll 2
+        for (int k0 = 0; k0 < kmax; k0 += 4) {
+            const int kk = k0 + lk;
+            const double2 va = gl_at(V, ar, kk, N);
+            const double2 ra = lds_at(sR, ar, kk, N, LD);
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int bc = (tj0 + t) * 16 + li;
+                const double2 vb = gl_at(V, kk, bc, N);
+                const double2 rb = lds_at(sR, kk, bc, N, LD);
+                cr[t] = mfma4(va.x, rb.x, cr[t]);
+                ci[t] = mfma4(va.x, rb.y, ci[t]);
+                cr[t] = mfma4(-va.y, rb.y, cr[t]);
+                ci[t] = mfma4(va.y, rb.x, ci[t]);
+                cr[t] = mfma4(ra.x, -vb.x, cr[t]);
+                ci[t] = mfma4(ra.y, -vb.x, ci[t]);
+                cr[t] = mfma4(ra.y, vb.y, cr[t]);
+                ci[t] = mfma4(ra.x, -vb.y, ci[t]);
+            }
+        }
+    }
+    // the stage update of the lane's 8 elements
+    double2 rg = make_double2(0.0, 0.0);
+    const bool rdo = st == 3 && a0 == 0 && readout_g(J, I, step, rg);
+    const double2* wp = st == 3 ? A.acc : ((st == 1 || st == 2) ? A.y : A.rin);
+    double2 wv[2][4];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int col = (tj0 + t) * 16 + li;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = ti * 16 + lk + 4 * r;
+            const bool in = row < N && col < N;
+            wv[t][r] = wp[(size_t)(base + (in ? row * N + col : 0))];
+        }
+    }
+    double2 qsum = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int col = (tj0 + t) * 16 + li;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = ti * 16 + lk + 4 * r;
+            if (row < N && col < N) {
+                const double2 x = sR[row * LD + col];
+                double ur = -ci[t][r], ui = cr[t][r];   // +i * C'
+                ur = fma(g.x, x.x, fma(-g.y, x.y, ur));
+                ui = fma(g.x, x.y, fma(g.y, x.x, ui));
+                const double2 cpl = coupling_of(I, sC, row, col);
+                const double2 yv = Yf[row * N + col];
+                const double2 qv = cconj_mul(x, yv);
+                qsum.x += fs * qv.x;
+                qsum.y += fs * qv.y;
+                if (rdo) readout_fold(J, I, ii, row, col, yv, rg, ur, ui);
+                rk_store_adj(A, I, (size_t)(base + row * N + col), x, wv[t][r], wv[t][r], ur + cpl.x, ui + cpl.y);
+            }
+        }
+    }
+    // Gt = Y K^H - K^H Y (K = fs l): P1[a][b] = sum_c Y[a][c] conj(K[b][c]),
+    // P2[a][b] = sum_c conj(K[c][a]) Y[c][b]
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        cr[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+        ci[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+    }
+#pragma unroll 2
+    for (int k0 = 0; k0 < kmax; k0 += 4) {
+        const int kk = k0 + lk;
+        const double2 y1 = gl_at(Yf, ar, kk, N);        // Y[a][c]   (A operand)
+        const double2 k2 = lds_at(sR, kk, ar, N, LD);   // l[c][a]   (A operand, conjugated)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int bc = (tj0 + t) * 16 + li;
+            const double2 k1 = lds_at(sR, bc, kk, N, LD);   // l[b][c]  (B operand, conjugated)
+            const double2 y2 = gl_at(Yf, kk, bc, N);        // Y[c][b]  (B operand)
+            cr[t] = mfma4(y1.x, k1.x, cr[t]);
+            ci[t] = mfma4(-y1.x, k1.y, ci[t]);
+            cr[t] = mfma4(y1.y, k1.y, cr[t]);
+            ci[t] = mfma4(y1.y, k1.x, ci[t]);
+            cr[t] = mfma4(-k2.x, y2.x, cr[t]);
+            ci[t] = mfma4(-k2.x, y2.y, ci[t]);
+            cr[t] = mfma4(-k2.y, y2.y, cr[t]);
+            ci[t] = mfma4(k2.y, y2.x, ci[t]);
+        }
+    }
+    double2* acc = J.gacc + J.gacc_off[ii] + (long long)(tile % J.ncopy) * gacc_size(NN, d);
+    const double2* ef = J.field + 3 * (size_t)(I.step_off + step);
+    const double2 e0 = ef[0], e1 = ef[1], e2 = ef[2];
+    const bool field_on = e0.x != 0.0 || e0.y != 0.0 || e1.x != 0.0 || e1.y != 0.0 || e2.x != 0.0 || e2.y != 0.0;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int col = (tj0 + t) * 16 + li;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = ti * 16 + lk + 4 * r;
+            if (row < N && col < N) {
+                const int ab = row * N + col;
+                const double2 gt = make_double2(fs * cr[t][r], fs * ci[t][r]);
+                atomic_add2(acc + ab, gt);
+                if (field_on) {
+                    atomic_add2(acc + NN + ab, cmad(make_double2(0.0, 0.0), e0, gt));
+                    atomic_add2(acc + 2 * NN + ab, cmad(make_double2(0.0, 0.0), e1, gt));
+                    atomic_add2(acc + 3 * NN + ab, cmad(make_double2(0.0, 0.0), e2, gt));
+                }
+            }
+        }
+    }
+    // q_n = fs sum conj(l) Y over the ADO, then n_p q_n; the bath reductions into sU
+    for (int o = 32; o > 0; o >>= 1) {
+        qsum.x += __shfl_xor(qsum.x, o, 64);
+        qsum.y += __shfl_xor(qsum.y, o, 64);
+    }
+    if (lane == 0) red[wave] = qsum;
+    reduce_bath_ld(J, I, a0, sR, LD, fs, sU, threadIdx.x, 512);
+    __syncthreads();
+    double2 qn = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int w = 0; w < 8; ++w) {
+        qn.x += red[w].x;
+        qn.y += red[w].y;
+    }
+    for (int p = threadIdx.x; p < d; p += 512) {
+        const int np_ = J.ado[I.lk_off + (long long)a0 * d + p];
+        if (np_) atomic_add2(acc + 5 * NN + p, make_double2(np_ * qn.x, np_ * qn.y));
+    }
+    for (int t = threadIdx.x; t < 3 * d; t += 512) {
+        const int w = t / d, p = t - w * d;
+        atomic_add2(acc + 5 * NN + (1 + w) * d + p, make_double2(sU[2 * (w * d + p)], sU[2 * (w * d + p) + 1]));
+    }
+}
+
 struct AdjVhArgs {
     const InstDev* inst;
     const int* step_ctr;
@@ -654,6 +888,17 @@ static size_t adj_generic_lds(const Group& g) {
     return (size_t)(2 * g.maxnn + g.maxkn) * sizeof(double2) + (size_t)6 * g.maxd * sizeof(double);
 }
 
+// dynamic LDS of adj_mfma64: l at stride N + 1, the coupling rows/columns, six d-vectors
+static size_t adj_mfma64_lds(const Group& g) {
+    int mn = 1;
+    while (mn * mn < g.maxnn) mn++;
+    return (size_t)(mn * (mn + 1) + g.maxkn) * sizeof(double2) + (size_t)6 * g.maxd * sizeof(double);
+}
+
+// the adjoint kernel of a group: adj_small for N <= 16, adj_mfma64 where the forward's
+// base kernel is stage_mfma64 (32 < N <= 64), adj_generic otherwise
+static bool adj_uses_mfma64(const Group& g) { return g.nt == -65; }
+
 static int launch_adj(const Group& g, const AdjArgs& base, hipStream_t s) {
     AdjArgs a = base;
     a.S.tile_inst += g.tile_begin;
@@ -669,7 +914,10 @@ static int launch_adj(const Group& g, const AdjArgs& base, hipStream_t s) {
 #undef EH_CASE
         default: {
             // every N > 16 group has one ADO per tile
-            hipLaunchKernelGGL(adj_generic, grid, block, adj_generic_lds(g), s, a);
+            if (adj_uses_mfma64(g))
+                hipLaunchKernelGGL(adj_mfma64, grid, dim3(512), adj_mfma64_lds(g), s, a);
+            else
+                hipLaunchKernelGGL(adj_generic, grid, block, adj_generic_lds(g), s, a);
             break;
         }
     }
@@ -733,14 +981,19 @@ int batch_enable_adjoint(Batch* b, int C) {
             stot += gacc_size(I.N * I.N, I.d);
         }
     }
-    // the N > 16 adjoint kernel stages rho, lambda and the coupling rows/columns in LDS
+    // the N > 16 adjoint kernels stage lambda (adj_generic: also rho) and the coupling
+    // rows/columns in LDS
     for (const Group& g : im->groups) {
         if (g.nt >= 1 && g.nt <= 16) continue;
-        const size_t sh = adj_generic_lds(g);
+        const bool mf = adj_uses_mfma64(g);
+        const size_t sh = mf ? adj_mfma64_lds(g) : adj_generic_lds(g);
         if ((long long)sh > b->ctx->lds_per_block)
-            return fail(EH_ESHAPE, "adjoint: an N > 16 instance needs " + std::to_string(sh) +
-                                       " bytes of LDS per workgroup (2 N^2 + 2 K N complex + 48 d); the device has " +
-                                       std::to_string(b->ctx->lds_per_block));
+            return fail(EH_ESHAPE, "adjoint: an N > 16 instance needs " + std::to_string(sh) + " bytes of LDS per " +
+                                       "workgroup (" + (mf ? "N (N+1)" : "2 N^2") + " + 2 K N complex + 48 d); " +
+                                       "the device has " + std::to_string(b->ctx->lds_per_block));
+        if (sh > 65536)
+            HIPCHK(hipFuncSetAttribute(mf ? (const void*)adj_mfma64 : (const void*)adj_generic,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
     }
     int rc;
     const size_t T = (size_t)a->total;

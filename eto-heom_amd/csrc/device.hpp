@@ -26,6 +26,8 @@ struct InstDev {
     double h2, h, hb[4];
     long long col_off;     // complex offset of ADO 0 in the coupled-column copies (K*N per ADO)
     long long vp_off;      // offset of the zero-padded 64x64 real/imag planes of V (N > 32), else -1
+    int hsym;              // H is real and symmetric (then so is V in the steps with no pulse on)
+    int pad_;
 };
 
 struct StageArgs {
@@ -358,6 +360,21 @@ __device__ __forceinline__ double2 coupling_of(const InstDev& I, const double2* 
         c.y += v.y;
     }
     return c;
+}
+
+// rho_n[r][c] from its N x (N+1) LDS copy, 0 in the padding up to 64 (clamped read, selected)
+__device__ __forceinline__ double2 lds_at(const double2* sR, int r, int c, int N, int LD) {
+    const bool in = r < N && c < N;
+    const double2 v = sR[(in ? r : 0) * LD + (in ? c : 0)];
+    return in ? v : make_double2(0.0, 0.0);
+}
+
+// element (r, c) of an N x N complex matrix in global memory, 0 beyond N (clamped address,
+// selected result: an unconditional load)
+__device__ __forceinline__ double2 gl_at(const double2* M, int r, int c, int N) {
+    const bool in = r < N && c < N;
+    const double2 v = M[in ? r * N + c : 0];
+    return in ? v : make_double2(0.0, 0.0);
 }
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));

@@ -207,9 +207,9 @@ def test_adjoint_with_pipelined_forward_candidate(ctx):
 
 
 def test_adjoint_chain_n56(ctx):
-    """N = 56, the 10-site chain's system (32 < N <= 64: the stage_mfma64 forward, the
-    generic adjoint kernel): site energies, a nearest-neighbour coupling, a
-    double-exciton entry and a dipole entry against FD."""
+    """N = 56, the 10-site chain's system (32 < N <= 64: the stage_mfma64 forward and the
+    adj_mfma64 adjoint): site energies, a nearest-neighbour coupling, a double-exciton
+    entry, a dipole entry and bath-mode parameters (the bath reductions) against FD."""
     p = configs.chain(L=1, nsteps=24)
     p.pulses[:, 1] = [2.0, 5.0, 8.0]
     p.pulses[:, 2] = 3.0
@@ -225,10 +225,40 @@ def test_adjoint_chain_n56(ctx):
     b.backward([w])
     g = b.grad(0)
     for name, idx, part in [("H", (1, 1), "re"), ("H", (6, 6), "re"), ("H", (1, 2), "re"), ("H", (11, 12), "re"),
-                            ("X", (0, 0, 4), "re")]:
+                            ("X", (0, 0, 4), "re"), ("gamma", (0, 0), "re"), ("gamma", (3, 1), "im"),
+                            ("alpha", (2, 0), "re"), ("alpha_t", (5, 1), "im")]:
         ref = fd(p, w, name, idx, part)
         got = adj_value(g, name, idx, part)
         assert abs(got - ref) / max(abs(ref), 1e-12) < 1e-6, (name, idx, got, ref)
+
+
+def test_adjoint_chain_mfma_equals_generic(ctx, monkeypatch):
+    """The N = 56 chain at L = 2 (231 ADOs) through two pulse windows: every gradient of
+    the MFMA adjoint kernel equals the VALU adjoint kernel's (selected by turning the MFMA
+    forward off) to 1e-10 of its scale (the two sum in different orders)."""
+    p = configs.chain(L=2, nsteps=40)
+    p.pulses[:, 1] = [2.0, 5.0, 8.0]
+    p.pulses[:, 2] = 3.0
+    p.pulses[:, 0] *= 3e5
+    rng = np.random.default_rng(5)
+    out = []
+    for no_mfma in ("0", "1"):
+        monkeypatch.setenv("ETOHEOM_NO_MFMA", no_mfma)
+        b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)}], adjoint=8)
+        kinds = {k for _, _, k, _ in b.candidates()}
+        assert ("stage_mfma64" in kinds) == (no_mfma == "0")
+        b.run()
+        b.sync()
+        P = b.output(0)
+        w = rng.normal(size=len(P)) + 1j * rng.normal(size=len(P)) if not out else out[0][2]
+        b.backward([w])
+        out.append((b.grad(0), P, w))
+        b.close()
+    assert np.max(np.abs(out[0][1] - out[1][1])) <= 1e-10 * np.max(np.abs(out[1][1]))
+    for name in ("H", "X", "alpha", "alpha_t", "gamma"):
+        a, c = out[0][0][name], out[1][0][name]
+        assert np.max(np.abs(c)) > 0, name
+        assert np.max(np.abs(a - c)) <= 1e-10 * np.max(np.abs(c)), (name, np.max(np.abs(a - c)) / np.max(np.abs(c)))
 
 
 def test_adjoint_replay_mode_large_n(ctx, monkeypatch):
