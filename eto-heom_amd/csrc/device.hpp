@@ -27,7 +27,7 @@ struct InstDev {
     long long col_off;     // complex offset of ADO 0 in the coupled-column copies (K*N per ADO)
     long long vp_off;      // offset of the zero-padded 64x64 real/imag planes of V (N > 32), else -1
     int hsym;              // H is real and symmetric (then so is V in the steps with no pulse on)
-    int pad_;
+    int sp_off;            // row offset of this instance's sparsity pattern of V (sp_cnt, sp_idx)
 };
 
 struct StageArgs {
@@ -75,7 +75,14 @@ struct StageArgs {
     // same L2 traffic and hit rate), 12 % slower on one that nearly fits it (the next stage
     // reads its input back from that cache).  The MFMA kernels ignore it.
     int stream;
+    // Sparsity pattern of V(t) over the whole run, per instance (row offset InstDev::sp_off):
+    // row a holds sp_cnt[a] column indices sp_idx[a * kSpCap + r] (ascending, -1 beyond);
+    // the pattern is symmetric, so row c's list is also column c's.
+    const int* sp_cnt;
+    const short* sp_idx;
 };
+
+constexpr int kSpCap = 16;   // entries per row of the sparsity pattern (the sparse kernel's limit)
 
 struct FusedDev {
     double2* V;              // copy 0 of V and of Vpad (the odd steps' copies vstride / vpstride on)
@@ -114,6 +121,18 @@ __device__ __forceinline__ void st_state(const StageArgs& A, double2* p, double2
     } else {
         *p = v;
     }
+}
+
+// A read of a table the kernel never writes (instance descriptors, tile lists, sparsity
+// patterns, this step's V), through the constant address space: with a uniform index it
+// is a scalar load.
+template <class T>
+__device__ __forceinline__ T ro(const T* p, long long i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ((const __attribute__((address_space(4))) T*)p)[i];
+#else
+    return p[i];   // host pass: never called
+#endif
 }
 
 __device__ __forceinline__ double2 cmad(double2 acc, double2 a, double2 b) {
@@ -436,11 +455,11 @@ __device__ __forceinline__ void build_v(const InstDev& I, const double2* H, cons
 }
 
 // The readouts Tr(rho_0 R_r) before a step (dynamics.cpp:367-381, :246-253) into
-// P[r * npoints + slot], by lanes 0..63 of the block (wave 0, every lane present).
+// P[r * npoints + slot], by the 64 lanes of one wave (lane = 0..63, every lane present).
 // One summation order wherever it runs, so every path gives the same bits.
 __device__ __forceinline__ void readout_wave(const InstDev& I, const double2* rho0, const double2* R, int slot,
-                                             double2* P) {
-    const int N = I.N, NN = N * N, lane = threadIdx.x;
+                                             double2* P, int lane = (int)threadIdx.x) {
+    const int N = I.N, NN = N * N;
     for (int r = 0; r < I.nread; ++r) {
         const double2* Rr = R + (size_t)r * NN;
         double2 s = make_double2(0.0, 0.0);
@@ -456,8 +475,8 @@ __device__ __forceinline__ void readout_wave(const InstDev& I, const double2* rh
     }
 }
 
-// End of a fused stage, in the workgroup whose tile starts at the instance's ADO 0
-// (threads tid = 0..nthr-1 of it, wave 0 among them):
+// End of a fused stage, in the workgroup (or the group of waves) whose tile starts at
+// the instance's ADO 0 (threads tid = 0..nthr-1 of it; tid 0..63 is one wave):
 // stage 0 reads out rho_0 before the step (its input, which no stage-0 workgroup
 // writes); stage 3 builds V of the next step into the copy the step after reads
 // (that copy was last read by the previous step's stages, all finished).
@@ -465,7 +484,7 @@ __device__ __forceinline__ void fused_tail(const StageArgs& A, const InstDev& I,
     const FusedDev& F = *A.fz;
     if (A.stage == 0) {
         const int slot = F.pidx[I.step_off + step];
-        if (slot >= 0 && threadIdx.x < 64) readout_wave(I, A.rin + I.state_off, F.R + I.r_off, slot, F.P + I.p_off);
+        if (slot >= 0 && tid < 64) readout_wave(I, A.rin + I.state_off, F.R + I.r_off, slot, F.P + I.p_off, tid);
     } else if (A.stage == 3 && step + 1 < I.nsteps) {
         const long long o = ((step + 1) & 1) ? 1 : 0;
         build_v(I, F.H + I.v_off, F.Xp + 3 * (size_t)I.v_off, F.field + 3 * (size_t)(I.step_off + step + 1),

@@ -538,6 +538,163 @@ __global__ __launch_bounds__(512, 2) void stage_mfma64(StageArgs A) {
     mfma64_tile<true>(A, blockIdx.x, step);
 }
 
+// ---------------------------------------------------------------------------------
+// Sparse commutator stage for 16 < N <= 64.  V(t) = H - sum_q e_q(t) X'_q keeps the
+// sparsity pattern of H and the three field operators over the whole run (made symmetric
+// on the host: the pattern of column c is that of row c), and for aggregates it is thin:
+// a double-exciton state couples to the doubles that share one site with it and to its
+// two singles (the 10-site chain, N = 56: at most 13 entries per row, 7.4 on average).
+// The commutator is then
+//   (V rho)[a][b] - (rho V)[a][b] = sum_{c in P(a)} V[a][c] rho[c][b] - sum_{c in P(b)} rho[a][c] V[c][b],
+// about 7x fewer flops than the dense product at N = 56, on the VALU (FP64 FMA runs at the
+// rate of FP64 MFMA on gfx950, and MFMA work holds up the other waves' VALU work), so the
+// launch is bound by its state traffic instead of by memory time plus MFMA time.
+// One ADO per workgroup of 512 threads: lane b of wave w computes column b of rows
+// a = w, w + 8, ...  rho_n (stride N + 1) and the coupling rows/columns (gather_couplings)
+// sit in LDS as in stage_mfma64, with each column's pattern: its row indices and
+// V[c][b].  V real and symmetric (no pulse on): row a's entries are column a's (one LDS
+// broadcast per entry).  In the pulse windows (V complex, not symmetric) a workgroup runs
+// stage_mfma64's dense tile instead.
+// Summation order: row pattern ascending, then column pattern ascending (exact zeros of
+// the dense product are skipped; the sum differs from it by rounding only).
+__global__ __launch_bounds__(512, 4) void stage_sparse(StageArgs A) {
+    constexpr int CAP = kSpCap, PS = kSpCap + 1;
+    extern __shared__ double2 smem[];
+    const int step = *A.step_ctr + A.step_off;
+    at_step(A, step);
+    const int tile = blockIdx.x;
+    const InstDev I = ro(A.inst, ro(A.tile_inst, tile));   // scalar loads, then registers
+    if (step >= I.nsteps) return;
+    const int N = I.N, NN = N * N, LD = N + 1, st = A.stage;
+    const int a0 = ro(A.tile_ado0, tile);
+    const long long base = I.state_off + (long long)a0 * NN;
+    double2* sR = smem;                                              // rho_n, N x (N+1)
+    double2* sC = sR + N * LD;                                       // coupling rows/columns, 2 K N
+    double* sVc = reinterpret_cast<double*>(sC + 2 * I.K * N);       // [N][PS]: Re V[c][b]
+    short* sIx = reinterpret_cast<short*>(sVc + N * PS);             // [N][PS]: c (-1 beyond)
+    int* sCnt = reinterpret_cast<int*>(reinterpret_cast<char*>(sIx) + ((N * PS * 2 + 3) & ~3));   // [N]
+    const int tid = threadIdx.x, b = tid & 63, w = tid >> 6;
+    const double2* V = A.V + I.v_off;
+    const short* pat = A.sp_idx + (long long)I.sp_off * CAP;
+    const bool vreal = A.vreal[I.step_off + step] != 0;
+    const bool sym = vreal && I.hsym;
+    // V complex or not symmetric (a pulse window): the dense MFMA tile, whose LDS layout
+    // (rho at stride N + 1, then the coupling rows/columns) is this kernel's first part
+    if (!sym) {
+        mfma64_tile<true>(A, tile, step);
+        return;
+    }
+    // 1. the stage input, the column patterns (N CAP <= 1024 entries: two per thread),
+    // the coupling gathers; every load unconditional (clamped address, selected value)
+    double2 xr[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int e = tid + q * 512, r = e >> 6, c = e & 63;
+        const bool in = r < N && c < N;
+        const double2 t = A.rin[base + (in ? r * N + c : 0)];
+        xr[q] = in ? t : make_double2(0.0, 0.0);
+    }
+    int ix[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) ix[h] = pat[min(tid + h * 512, N * CAP - 1)];
+    const int cnt = A.sp_cnt[I.sp_off + min(tid, N - 1)];
+    gather_couplings(A, I, a0, 1, sC, tid, 512);
+    double vc[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int e = min(tid + h * 512, N * CAP - 1), c = e / CAP;
+        vc[h] = V[(ix[h] >= 0 ? ix[h] : 0) * N + c].x;   // Re V[row][c] of column c's entry
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int e = tid + q * 512, r = e >> 6, c = e & 63;
+        if (r < N && c < N) sR[r * LD + c] = xr[q];
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int e = tid + h * 512;
+        if (e < N * CAP) {
+            const int c = e / CAP, r = e - c * CAP;
+            sIx[c * PS + r] = (short)ix[h];
+            sVc[c * PS + r] = ix[h] >= 0 ? vc[h] : 0.0;
+        }
+    }
+    if (tid < N) sCnt[tid] = cnt;
+    __syncthreads();
+    // 2. the Y / ACC operand of the lane's elements, then each row
+    const double2 g = ro(A.damp, I.ado_off + a0);
+    const double2* wp = st == 3 ? A.acc : A.y;
+    const int bc = b < N ? b : 0;
+    double2 wv[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int a = w + 8 * i;
+        const bool in = a < N && b < N;
+        wv[i] = st != 0 ? wp[base + (in ? a * N + b : 0)] : make_double2(0.0, 0.0);
+    }
+    // column b's pattern in registers (the same for every row of the lane); the loops over
+    // pattern entries run the full capacity (absent entries: index 0, weight 0) so that
+    // their LDS reads are issued together rather than one dependent pair at a time
+    // entries in row a (uniform per row) and the wave's largest column count: the loops
+    // below skip the entries beyond them (uniform branches)
+    int ncw = b < N ? sCnt[b] : 0;   // the pattern is symmetric: column b's count
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) ncw = max(ncw, __shfl_xor(ncw, o, 64));
+    ncw = __builtin_amdgcn_readfirstlane(ncw);
+    unsigned cix2[CAP / 2];   // two 16-bit indices per register
+    double cv[CAP];
+#pragma unroll
+    for (int r = 0; r < CAP; r += 2) {
+        const int c0 = sIx[bc * PS + r], c1 = sIx[bc * PS + r + 1];
+        cix2[r / 2] = (unsigned)(c0 >= 0 ? c0 : 0) | ((unsigned)(c1 >= 0 ? c1 : 0) << 16);
+    }
+#pragma unroll
+    for (int r = 0; r < CAP; ++r) cv[r] = sVc[bc * PS + r];   // 0 for an absent entry
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int a = w + 8 * i;
+        if (a >= N) break;   // uniform in the wave
+        double2 acc = make_double2(0.0, 0.0);
+        {
+            // (V rho)[a][b]: V is real and symmetric, so row a's entries are column a's
+            // (LDS broadcasts)
+            const int nra = __builtin_amdgcn_readfirstlane(sCnt[a]);
+#pragma unroll
+            for (int r = 0; r < CAP; ++r) {
+                if (r < nra) {
+                    const int c = sIx[a * PS + r];
+                    const double v = sVc[a * PS + r];
+                    const double2 x = sR[c * LD + bc];
+                    acc.x = fma(v, x.x, acc.x);
+                    acc.y = fma(v, x.y, acc.y);
+                }
+            }
+            // - (rho V)[a][b]: column b's entries (V real)
+#pragma unroll
+            for (int r = 0; r < CAP; ++r) {
+                if (r < ncw) {
+                    const int c = (int)((cix2[r / 2] >> (16 * (r & 1))) & 0xffffu);
+                    const double2 x = sR[a * LD + c];
+                    acc.x = fma(-x.x, cv[r], acc.x);
+                    acc.y = fma(-x.y, cv[r], acc.y);
+                }
+            }
+        }
+        if (b < N) {
+            const double2 x = sR[a * LD + b];
+            double dr = acc.y, di = -acc.x;   // -i * acc
+            dr = fma(g.x, x.x, fma(-g.y, x.y, dr));
+            di = fma(g.x, x.y, fma(g.y, x.x, di));
+            const double2 cpl = coupling_of(I, sC, a, b);
+            const double2 ww = wv[i];
+            const double2 y0 = st == 0 ? x : ww;
+            const double2 a1 = st == 1 ? x : (st == 3 ? make_double2(fma(x.x, 1.0 / 3.0, ww.x), fma(x.y, 1.0 / 3.0, ww.y)) : ww);
+            rk_store<false, true>(A, I, (size_t)(base + a * N + b), x, y0, a1, dr + cpl.x, di + cpl.y, a0, a, b);
+        }
+    }
+    if (A.fused && a0 == 0) fused_tail(A, I, step, tid, 512);
+}
+
 // Coupling term of output element (row, col) of the ADO whose table row starts at lrow,
 // gathered straight from the stage input: the same element of the up / down neighbours
 // of the bath whose site is row (weights i x, i alpha y) and of the bath whose site is
@@ -1042,6 +1199,7 @@ int launch_cfg(const Group& g, const LaunchCfg& c, const StageArgs& base, hipStr
         case -64: hipLaunchKernelGGL((stage_mfma<64, false>), grid, block, c.shmem, s, a); break;
         case -65: hipLaunchKernelGGL(stage_mfma64, grid, dim3(512), c.shmem, s, a); break;
         case -128: hipLaunchKernelGGL(stage_mfma_wide, grid, block, 0, s, a); break;
+        case -67: hipLaunchKernelGGL(stage_sparse, grid, dim3(512), c.shmem, s, a); break;
         case -33:
             // the RK coefficients of this stage are arguments (one time step per group)
             hipLaunchKernelGGL(stage_mfma_pipe<2>, dim3(std::min(c.tile_count, g.persist)), block, c.shmem, s, a,
@@ -1082,6 +1240,8 @@ StageArgs stage_args(const BatchImpl* im) {
     sa.alpha = im->alpha.p;
     sa.alpha_t = im->alpha_t.p;
     sa.damp = im->damp.p;
+    sa.sp_cnt = im->sp_cnt.p;
+    sa.sp_idx = im->sp_idx.p;
     sa.gsign = 1.0;
     sa.y = im->Y.p;
     sa.acc = im->ACC.p;
@@ -1195,6 +1355,10 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
     // distinct (d, L, alpha): they depend on nothing else, so a grid of propagations
     // with one bath shares a few small, cache-resident tables.
     std::map<std::vector<double>, int64_t> hier_off;
+    std::map<std::vector<short>, int> sp_rows;   // distinct sparsity patterns of V -> first row
+    std::vector<int> isp((size_t)n, 0);          // largest pattern row count per instance
+    hb.spcnt.clear();
+    hb.spidx.clear();
     std::vector<uint8_t>& hvreal = hb.vreal;
     std::vector<double2>&hH = hb.H, &hXp = hb.Xp, &hfield = hb.field, &hR = hb.R, &hal = hb.al, &halt = hb.alt,
                          &hdamp = hb.damp;
@@ -1257,6 +1421,38 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
                     const int e = qq == 0 ? a * N + bb : bb * N + a;   // X[1], X[2] transposed
                     hXp.push_back(make_double2(q.X4[2 * (qq * NN + e)], q.X4[2 * (qq * NN + e) + 1]));
                 }
+        // the sparsity pattern of V(t) = H - sum_q e_q(t) X'_q over the whole run, made
+        // symmetric; instances with the same pattern share its rows
+        int sp_row = -1, sp_max = 0;
+        {
+            const double2* Hh = hH.data() + (size_t)v_off;
+            const double2* Xh = hXp.data() + (size_t)3 * v_off;
+            auto z = [](double2 v) { return v.x != 0.0 || v.y != 0.0; };
+            std::vector<short> pat((size_t)N * kSpCap, (short)-1);
+            std::vector<int> cnt((size_t)N, 0);
+            for (int a = 0; a < N; a++) {
+                for (int c = 0; c < N; c++) {
+                    bool on = z(Hh[a * N + c]) || z(Hh[c * N + a]);
+                    for (int qq = 0; qq < 3 && !on; qq++) on = z(Xh[qq * NN + a * N + c]) || z(Xh[qq * NN + c * N + a]);
+                    if (!on) continue;
+                    if (cnt[(size_t)a] < kSpCap) pat[(size_t)a * kSpCap + cnt[(size_t)a]] = (short)c;
+                    cnt[(size_t)a]++;
+                }
+                sp_max = std::max(sp_max, cnt[(size_t)a]);
+            }
+            if (sp_max <= kSpCap) {
+                const auto it = sp_rows.find(pat);
+                if (it != sp_rows.end()) {
+                    sp_row = it->second;
+                } else {
+                    sp_row = (int)hb.spcnt.size();
+                    sp_rows[pat] = sp_row;
+                    hb.spcnt.insert(hb.spcnt.end(), cnt.begin(), cnt.end());
+                    hb.spidx.insert(hb.spidx.end(), pat.begin(), pat.end());
+                }
+            }
+        }
+        isp[(size_t)i] = sp_max;
         for (int st = 0; st < s.nsteps; st++) {
             for (int qq = 0; qq < 3; qq++) hfield.push_back(d2(s.e[(size_t)3 * st + qq]));
             hpidx.push_back(s.pidx[(size_t)st]);
@@ -1306,6 +1502,7 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
         d.col_off = col_off;
         col_off += (int64_t)M * sys.K * N;
         d.hsym = real_h && sym_h ? 1 : 0;
+        d.sp_off = sp_row;
         d.vp_off = -1;
         if (N > 32 && N <= 64) {
             d.vp_off = vp_off;
@@ -1382,7 +1579,7 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
         g.inst_count = (int)members.size();
         hginst.insert(hginst.end(), members.begin(), members.end());
         g.persist = 2 * ctx->n_cu;
-        int maxnn = 0, maxkn = 0, maxd = 0;
+        int maxnn = 0, maxkn = 0, maxd = 0, maxsp = 0;
         bool pipe_ok = cls == 17;
         bool tp_ok = true;   // two-phase gathers: 2 K N <= 512 and K_m <= 2 (and a bath) everywhere
         const InstDev& f0 = hinst[(size_t)members[0]];
@@ -1391,6 +1588,7 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
             maxnn = std::max(maxnn, di.N * di.N);
             maxkn = std::max(maxkn, 2 * di.K * di.N);
             maxd = std::max(maxd, di.d);
+            maxsp = std::max(maxsp, isp[(size_t)i]);
             // the pipelined kernel takes the RK coefficients as arguments: one time step per group
             if (di.Km > 2 || 2 * di.K * di.N > 512 || di.h != f0.h) pipe_ok = false;
             if (di.Km > 2 || di.Km < 1 || di.K < 1 || 2 * di.K * di.N > 512) tp_ok = false;
@@ -1398,6 +1596,9 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
         g.maxnn = maxnn;
         g.maxkn = maxkn;
         g.maxd = maxd;
+        g.maxsp = maxsp;
+        // the sparse-commutator kernel: every instance's pattern within its row capacity
+        const bool sparse_ok = maxsp <= kSpCap && !env_on("ETOHEOM_NO_SPARSE");
         // Non-temporal state stores in the small-N kernels by default when the group's four
         // state buffers exceed 512 MB, twice the last-level cache (measured on the 2D grid:
         // the 1.6 GB N = 11 group's stage 3 % faster with them, the 224 MB N = 7 group's
@@ -1456,6 +1657,12 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
             }
             else if (nt == -128)
                 c.shmem = 0;   // stage_mfma_wide: nothing in LDS
+            else if (nt == -67) {   // stage_sparse: rho, couplings, column patterns
+                int mn = 1;
+                while (mn * mn < maxnn) mn++;
+                c.shmem = (size_t)mn * (mn + 1) * sizeof(double2) + cpb + (size_t)mn * (kSpCap + 1) * sizeof(double) +
+                          (((size_t)mn * (kSpCap + 1) * sizeof(short) + 3) & ~(size_t)3) + (size_t)mn * sizeof(int);
+            }
             else if (nt > 0)
                 c.shmem = (size_t)G * cpb;   // the coupling rows/columns of G ADOs (the rest is static)
             else
@@ -1464,6 +1671,7 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
             if (c.shmem > 65536) {
                 const void* fn = nt == -64   ? (const void*)stage_mfma<64, false>
                                  : nt == -65 ? (const void*)stage_mfma64
+                                 : nt == -67 ? (const void*)stage_sparse
                                  : (nt == -33 ? (const void*)stage_mfma_pipe<2> : (const void*)stage_generic);
                 if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.shmem) != hipSuccess)
                     return;
@@ -1488,6 +1696,7 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
             if (use_mfma) add(-128, 0, false, stream);
             add(0, 0, false, stream);
         } else {   // 32 < N <= 64: stage_mfma64 gathers columns directly (measured faster at N = 56)
+            if (sparse_ok) add(-67, 0, false, stream);
             if (use_mfma) add(-65, 0, false, stream);
             add(0, 0, true, stream);
             if (use_mfma) {
@@ -1530,7 +1739,8 @@ int batch_create(eh_ctx* ctx, int n, const eh_instance* in, Batch** out) {
         (rc = im->cy.upload(hb.cy)) || (rc = im->pidx.upload(hb.pidx)) || (rc = im->vreal.upload(hb.vreal)) ||
         (rc = im->H.upload(hb.H)) || (rc = im->Xp.upload(hb.Xp)) || (rc = im->field.upload(hb.field)) ||
         (rc = im->R.upload(hb.R)) || (rc = im->alpha.upload(hb.al)) || (rc = im->alpha_t.upload(hb.alt)) ||
-        (rc = im->damp.upload(hb.damp)) || (rc = im->ado0_off.upload(hb.ado0)) || (rc = im->ginst.upload(hb.ginst)))
+        (rc = im->damp.upload(hb.damp)) || (rc = im->ado0_off.upload(hb.ado0)) || (rc = im->ginst.upload(hb.ginst)) ||
+        (rc = im->sp_cnt.upload(hb.spcnt)) || (rc = im->sp_idx.upload(hb.spidx)))
         return rc;
     if ((rc = im->step_ctr.alloc(1)) || (rc = im->V.alloc(2 * hb.H.size())) || (rc = im->P.alloc((size_t)p_off)) ||
         (rc = im->Y.alloc((size_t)state_off)) || (rc = im->ACC.alloc((size_t)state_off)) ||
@@ -1830,6 +2040,7 @@ std::string batch_tuning_report(const Batch* b) {
                              : x.nt == -34 ? "stage_mfma32_tp"
                              : x.nt == -64 ? "stage_mfma64_v1"
                              : x.nt == -65 ? "stage_mfma64"
+                             : x.nt == -67 ? "stage_sparse"
                              : x.nt == -33 ? "stage_mfma_pipe"
                              : x.nt == -128 ? "stage_mfma_wide"
                                             : "stage_generic";

@@ -53,7 +53,8 @@ struct DevBuf {
 // One way of launching a group's stage: kernel variant and tiling.
 struct LaunchCfg {
     int nt;                   // 1..16: stage_small<nt>; -32 / -64: stage_mfma; -65: stage_mfma64;
-                              // -33: stage_mfma_pipe; -128: stage_mfma_wide; 0: stage_generic
+                              // -33: stage_mfma_pipe; -128: stage_mfma_wide; -67: stage_sparse;
+                              // 0: stage_generic
     int ge;                   // stage_small: state elements per tile (256, 512 or 1024)
     bool cols;                // keep coupled-column copies (contiguous column gathers)
     bool stream;              // non-temporal state stores (StageArgs::stream)
@@ -69,6 +70,7 @@ struct Group {
     int tile_begin, tile_count;
     size_t shmem;
     int maxnn, maxkn, maxd;   // largest N^2, 2 K N and d of the group's instances
+    int maxsp;                // largest row count of the instances' sparsity patterns of V
     int persist;              // grid of the persistent kernels (workgroups)
     double h, h2, hb[4];      // RK coefficients of the pipelined variant (one time step)
     int inst_begin, inst_count;   // the group's instances, a range of BatchImpl::ginst
@@ -91,6 +93,8 @@ struct HostBatch {
     int64_t p_elems = 0;       // complex readouts
     int64_t col_elems = 0;     // complex elements of each coupled-column copy
     int64_t vp_elems = 0;      // doubles of one copy of the padded V planes
+    std::vector<int> spcnt;    // sparsity pattern of V (rows of distinct patterns, InstDev::sp_off)
+    std::vector<short> spidx;
 };
 int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBatch& hb);
 // reset_state + set_ground of engine.hip on raw buffers: every state element and
@@ -119,6 +123,8 @@ struct BatchImpl {
     DevBuf<double2> Yc, S1c, S2c;        // coupled-column copies of Y, S1, S2
     DevBuf<double> Vpad;                 // padded V planes of the N > 32 instances
     DevBuf<long long> ado0_off;
+    DevBuf<int> sp_cnt;                  // sparsity pattern of V per instance row (StageArgs::sp_cnt)
+    DevBuf<short> sp_idx;
     DevBuf<int> ginst;                   // instance indices, grouped
     std::vector<Group> groups;
     std::vector<hipStream_t> side;       // one capture branch per group beyond the first

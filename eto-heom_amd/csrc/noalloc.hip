@@ -89,7 +89,8 @@ int solve_2des_noalloc(eh_ctx* ctx, const eh_instance& in, double* P_out, int64_
                  o_al = cv.take<double2>(hb.al.size()), o_alt = cv.take<double2>(hb.alt.size()),
                  o_damp = cv.take<double2>(hb.damp.size()), o_ctr = cv.take<int>(1),
                  o_V = cv.take<double2>(hb.H.size()), o_Vpad = cv.take<double>((size_t)hb.vp_elems),
-                 o_P = cv.take<double2>((size_t)hb.p_elems);
+                 o_P = cv.take<double2>((size_t)hb.p_elems), o_spc = cv.take<int>(hb.spcnt.size()),
+                 o_spi = cv.take<short>(hb.spidx.size());
     size_t o_st[4], o_col[3];
     for (size_t& o : o_st) o = cv.take<double2>((size_t)hb.state_elems);
     for (size_t& o : o_col) o = cv.take<double2>((size_t)hb.col_elems);
@@ -112,7 +113,8 @@ int solve_2des_noalloc(eh_ctx* ctx, const eh_instance& in, double* P_out, int64_
     EH_PUT(o_inst, hb.inst) EH_PUT(o_ti, hb.t_inst) EH_PUT(o_ta, hb.t_ado0) EH_PUT(o_tn, hb.t_nado)
     EH_PUT(o_up, hb.up) EH_PUT(o_dn, hb.dn) EH_PUT(o_cx, hb.cx) EH_PUT(o_cy, hb.cy) EH_PUT(o_pidx, hb.pidx)
     EH_PUT(o_vreal, hb.vreal) EH_PUT(o_H, hb.H) EH_PUT(o_Xp, hb.Xp) EH_PUT(o_field, hb.field) EH_PUT(o_R, hb.R)
-    EH_PUT(o_al, hb.al) EH_PUT(o_alt, hb.alt) EH_PUT(o_damp, hb.damp)
+    EH_PUT(o_al, hb.al) EH_PUT(o_alt, hb.alt) EH_PUT(o_damp, hb.damp) EH_PUT(o_spc, hb.spcnt)
+    EH_PUT(o_spi, hb.spidx)
 #undef EH_PUT
     auto at = [&](size_t off) { return static_cast<void*>(base + off); };
     const InstDev* inst = static_cast<const InstDev*>(at(o_inst));
@@ -142,6 +144,8 @@ int solve_2des_noalloc(eh_ctx* ctx, const eh_instance& in, double* P_out, int64_
     sa.alpha = static_cast<const double2*>(at(o_al));
     sa.alpha_t = static_cast<const double2*>(at(o_alt));
     sa.damp = static_cast<const double2*>(at(o_damp));
+    sa.sp_cnt = static_cast<const int*>(at(o_spc));
+    sa.sp_idx = static_cast<const short*>(at(o_spi));
     sa.gsign = 1.0;
     sa.y = Y;
     sa.acc = ACC;
