@@ -896,8 +896,8 @@ static size_t adj_mfma64_lds(const Group& g) {
 }
 
 // the adjoint kernel of a group: adj_small for N <= 16, adj_mfma64 where the forward's
-// base kernel is stage_mfma64 or stage_sparse (32 < N <= 64), adj_generic otherwise
-static bool adj_uses_mfma64(const Group& g) { return g.nt == -65 || g.nt == -67; }
+// base kernel is stage_mfma64 (32 < N <= 64), adj_generic otherwise
+static bool adj_uses_mfma64(const Group& g) { return g.nt == -65; }
 
 static int launch_adj(const Group& g, const AdjArgs& base, hipStream_t s) {
     AdjArgs a = base;

@@ -539,7 +539,7 @@ __global__ __launch_bounds__(512, 2) void stage_mfma64(StageArgs A) {
 }
 
 // ---------------------------------------------------------------------------------
-// Sparse commutator stage for 16 < N <= 64.  V(t) = H - sum_q e_q(t) X'_q keeps the
+// Sparse commutator stage for 32 < N <= 64.  V(t) = H - sum_q e_q(t) X'_q keeps the
 // sparsity pattern of H and the three field operators over the whole run (made symmetric
 // on the host: the pattern of column c is that of row c), and for aggregates it is thin:
 // a double-exciton state couples to the doubles that share one site with it and to its
@@ -549,14 +549,15 @@ __global__ __launch_bounds__(512, 2) void stage_mfma64(StageArgs A) {
 // about 7x fewer flops than the dense product at N = 56, on the VALU (FP64 FMA runs at the
 // rate of FP64 MFMA on gfx950, and MFMA work holds up the other waves' VALU work), so the
 // launch is bound by its state traffic instead of by memory time plus MFMA time.
-// One ADO per workgroup of 512 threads: lane b of wave w computes column b of rows
-// a = w, w + 8, ...  rho_n (stride N + 1) and the coupling rows/columns (gather_couplings)
-// sit in LDS as in stage_mfma64, with each column's pattern: its row indices and
-// V[c][b].  V real and symmetric (no pulse on): row a's entries are column a's (one LDS
-// broadcast per entry).  In the pulse windows (V complex, not symmetric) a workgroup runs
-// stage_mfma64's dense tile instead.
-// Summation order: row pattern ascending, then column pattern ascending (exact zeros of
-// the dense product are skipped; the sum differs from it by rounding only).
+// One ADO per workgroup of 512 threads (two per CU): lane b of wave w computes column b
+// of rows a = w, w + 8, ...  rho_n (stride N + 1) and the coupling rows/columns
+// (gather_couplings) sit in LDS as in stage_mfma64, with each column's list: its row
+// indices c and Re V[c][b].  V real and symmetric (no pulse on): row a's entries are
+// column a's (LDS broadcasts).  Otherwise (the pulse windows) row a's values V[a][c] are
+// scalar loads (a and c are uniform in the wave) and Im V[c][b] of the lane's column is
+// loaded into registers once.  Summation order: row pattern ascending, then column
+// pattern ascending (the exact zeros of the dense product are skipped; the sum differs
+// from it by rounding only).
 __global__ __launch_bounds__(512, 4) void stage_sparse(StageArgs A) {
     constexpr int CAP = kSpCap, PS = kSpCap + 1;
     extern __shared__ double2 smem[];
@@ -570,22 +571,16 @@ __global__ __launch_bounds__(512, 4) void stage_sparse(StageArgs A) {
     const long long base = I.state_off + (long long)a0 * NN;
     double2* sR = smem;                                              // rho_n, N x (N+1)
     double2* sC = sR + N * LD;                                       // coupling rows/columns, 2 K N
-    double* sVc = reinterpret_cast<double*>(sC + 2 * I.K * N);       // [N][PS]: Re V[c][b]
-    short* sIx = reinterpret_cast<short*>(sVc + N * PS);             // [N][PS]: c (-1 beyond)
+    double* sVr = reinterpret_cast<double*>(sC + 2 * I.K * N);       // [N][PS]: Re V[c][b]
+    short* sIx = reinterpret_cast<short*>(sVr + N * PS);             // [N][PS]: c (-1 beyond)
     int* sCnt = reinterpret_cast<int*>(reinterpret_cast<char*>(sIx) + ((N * PS * 2 + 3) & ~3));   // [N]
     const int tid = threadIdx.x, b = tid & 63, w = tid >> 6;
     const double2* V = A.V + I.v_off;
-    const short* pat = A.sp_idx + (long long)I.sp_off * CAP;
+    const long long pbase = (long long)I.sp_off * CAP;
     const bool vreal = A.vreal[I.step_off + step] != 0;
     const bool sym = vreal && I.hsym;
-    // V complex or not symmetric (a pulse window): the dense MFMA tile, whose LDS layout
-    // (rho at stride N + 1, then the coupling rows/columns) is this kernel's first part
-    if (!sym) {
-        mfma64_tile<true>(A, tile, step);
-        return;
-    }
-    // 1. the stage input, the column patterns (N CAP <= 1024 entries: two per thread),
-    // the coupling gathers; every load unconditional (clamped address, selected value)
+    // 1. the stage input, the column lists (N CAP <= 1024 entries: two per thread), the
+    // coupling gathers; every load unconditional (clamped address, selected value)
     double2 xr[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -596,7 +591,7 @@ __global__ __launch_bounds__(512, 4) void stage_sparse(StageArgs A) {
     }
     int ix[2];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) ix[h] = pat[min(tid + h * 512, N * CAP - 1)];
+    for (int h = 0; h < 2; ++h) ix[h] = A.sp_idx[pbase + min(tid + h * 512, N * CAP - 1)];
     const int cnt = A.sp_cnt[I.sp_off + min(tid, N - 1)];
     gather_couplings(A, I, a0, 1, sC, tid, 512);
     double vc[2];
@@ -616,7 +611,7 @@ __global__ __launch_bounds__(512, 4) void stage_sparse(StageArgs A) {
         if (e < N * CAP) {
             const int c = e / CAP, r = e - c * CAP;
             sIx[c * PS + r] = (short)ix[h];
-            sVc[c * PS + r] = ix[h] >= 0 ? vc[h] : 0.0;
+            sVr[c * PS + r] = ix[h] >= 0 ? vc[h] : 0.0;
         }
     }
     if (tid < N) sCnt[tid] = cnt;
@@ -632,51 +627,67 @@ __global__ __launch_bounds__(512, 4) void stage_sparse(StageArgs A) {
         const bool in = a < N && b < N;
         wv[i] = st != 0 ? wp[base + (in ? a * N + b : 0)] : make_double2(0.0, 0.0);
     }
-    // column b's pattern in registers (the same for every row of the lane); the loops over
-    // pattern entries run the full capacity (absent entries: index 0, weight 0) so that
-    // their LDS reads are issued together rather than one dependent pair at a time
-    // entries in row a (uniform per row) and the wave's largest column count: the loops
-    // below skip the entries beyond them (uniform branches)
+    // the wave's largest column count: the column loops skip the entries beyond it (a
+    // uniform branch); column b's indices and values stay in registers (the same for
+    // every row of the lane)
     int ncw = b < N ? sCnt[b] : 0;   // the pattern is symmetric: column b's count
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) ncw = max(ncw, __shfl_xor(ncw, o, 64));
     ncw = __builtin_amdgcn_readfirstlane(ncw);
     unsigned cix2[CAP / 2];   // two 16-bit indices per register
-    double cv[CAP];
 #pragma unroll
     for (int r = 0; r < CAP; r += 2) {
         const int c0 = sIx[bc * PS + r], c1 = sIx[bc * PS + r + 1];
         cix2[r / 2] = (unsigned)(c0 >= 0 ? c0 : 0) | ((unsigned)(c1 >= 0 ? c1 : 0) << 16);
     }
+    auto cidx = [&](int r) { return (int)((cix2[r / 2] >> (16 * (r & 1))) & 0xffffu); };
+    // Re V[c][b] (real symmetric V) or Im V[c][b] (otherwise) of column b's entries
+    double cv[CAP];
 #pragma unroll
-    for (int r = 0; r < CAP; ++r) cv[r] = sVc[bc * PS + r];   // 0 for an absent entry
+    for (int r = 0; r < CAP; ++r)
+        cv[r] = sym ? sVr[bc * PS + r] : (sIx[bc * PS + r] >= 0 ? V[cidx(r) * N + bc].y : 0.0);
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         const int a = w + 8 * i;
         if (a >= N) break;   // uniform in the wave
+        const int nra = __builtin_amdgcn_readfirstlane(sCnt[a]);
         double2 acc = make_double2(0.0, 0.0);
-        {
-            // (V rho)[a][b]: V is real and symmetric, so row a's entries are column a's
-            // (LDS broadcasts)
-            const int nra = __builtin_amdgcn_readfirstlane(sCnt[a]);
+        if (sym) {
+            // (V rho)[a][b]: V real and symmetric, row a's entries are column a's (broadcasts)
 #pragma unroll
             for (int r = 0; r < CAP; ++r) {
                 if (r < nra) {
                     const int c = sIx[a * PS + r];
-                    const double v = sVc[a * PS + r];
+                    const double v = sVr[a * PS + r];
                     const double2 x = sR[c * LD + bc];
                     acc.x = fma(v, x.x, acc.x);
                     acc.y = fma(v, x.y, acc.y);
                 }
             }
-            // - (rho V)[a][b]: column b's entries (V real)
+            // - (rho V)[a][b]: column b's entries
 #pragma unroll
             for (int r = 0; r < CAP; ++r) {
                 if (r < ncw) {
-                    const int c = (int)((cix2[r / 2] >> (16 * (r & 1))) & 0xffffu);
-                    const double2 x = sR[a * LD + c];
+                    const double2 x = sR[a * LD + cidx(r)];
                     acc.x = fma(-x.x, cv[r], acc.x);
                     acc.y = fma(-x.y, cv[r], acc.y);
+                }
+            }
+        } else {
+            // (V rho)[a][b]: V[a][c] by scalar loads (row a and column c uniform)
+#pragma unroll
+            for (int r = 0; r < CAP; ++r) {
+                if (r < nra) {
+                    const int c = __builtin_amdgcn_readfirstlane((int)sIx[a * PS + r]);
+                    acc = cmad(acc, ro(V, (long long)a * N + c), sR[c * LD + bc]);
+                }
+            }
+            // - (rho V)[a][b]: column b's entries, V[c][b] = Re from LDS + i Im from registers
+#pragma unroll
+            for (int r = 0; r < CAP; ++r) {
+                if (r < ncw) {
+                    const double2 v = make_double2(-sVr[bc * PS + r], -cv[r]);
+                    acc = cmad(acc, sR[a * LD + cidx(r)], v);
                 }
             }
         }
@@ -1657,11 +1668,12 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
             }
             else if (nt == -128)
                 c.shmem = 0;   // stage_mfma_wide: nothing in LDS
-            else if (nt == -67) {   // stage_sparse: rho, couplings, column patterns
+            else if (nt == -67) {   // stage_sparse: rho, the couplings, the column lists
                 int mn = 1;
                 while (mn * mn < maxnn) mn++;
-                c.shmem = (size_t)mn * (mn + 1) * sizeof(double2) + cpb + (size_t)mn * (kSpCap + 1) * sizeof(double) +
-                          (((size_t)mn * (kSpCap + 1) * sizeof(short) + 3) & ~(size_t)3) + (size_t)mn * sizeof(int);
+                const size_t ps = kSpCap + 1;
+                c.shmem = (size_t)mn * (mn + 1) * sizeof(double2) + cpb + (size_t)mn * ps * sizeof(double) +
+                          (((size_t)mn * ps * sizeof(short) + 3) & ~(size_t)3) + (size_t)mn * sizeof(int);
             }
             else if (nt > 0)
                 c.shmem = (size_t)G * cpb;   // the coupling rows/columns of G ADOs (the rest is static)
@@ -1696,9 +1708,9 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
             if (use_mfma) add(-128, 0, false, stream);
             add(0, 0, false, stream);
         } else {   // 32 < N <= 64: stage_mfma64 gathers columns directly (measured faster at N = 56)
-            if (sparse_ok) add(-67, 0, false, stream);
             if (use_mfma) add(-65, 0, false, stream);
             add(0, 0, true, stream);
+            if (sparse_ok) add(-67, 0, false, stream);   // the default (below)
             if (use_mfma) {
                 add(-65, 0, true, stream);
                 add(-64, 0, true, stream);
@@ -1712,8 +1724,10 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
         }
         if (g.cand.empty() || (cls <= 16 && g.cand[0].nt != cls))
             return fail(EH_ESHAPE, "state tile exceeds LDS");
-        for (size_t c = 0; c < g.cand.size(); c++)
+        for (size_t c = 0; c < g.cand.size(); c++) {
             if (use_pipe && g.cand[c].nt == -33) g.sel = (int)c;
+            if (g.cand[c].nt == -67) g.sel = (int)c;   // measured: chain grid 25.5 -> 20.5 ms per stage
+        }
         g.nt = g.cand[0].nt;
         g.tile_begin = g.cand[0].tile_begin;
         g.tile_count = g.cand[0].tile_count;
