@@ -151,3 +151,39 @@ def test_chain_grid_gradient_in_sub_batches():
         g = out["one"][1][k]
         assert torch.all(torch.isfinite(g)) and torch.max(torch.abs(g)) > 0, k
         assert _rel(out["split"][1][k], g) < 1e-10, (k, _rel(out["split"][1][k], g))
+
+
+def test_trimer_grid_full_workload_directional_derivatives():
+    """The trimer grid at its full size (16 x 16 (t1, t3) rephasing points at t2 = 100 fs, 48
+    propagations, L = 5, checkpoints every 32 steps): the adjoint gradient projected on
+    random directions in (site energies, couplings, dipoles) and in (ETO damping,
+    reorganisation scale) equals the Richardson central difference of the same loss computed
+    by the GPU forward (which the oracle tests pin), relative <= 1e-6."""
+    ctx = E.Context(0)
+    spec = SP.grid_spec(ctx)
+    assert len(spec.nv) == 48 and spec.L == 5 and spec.checkpoint == 32
+    with torch.no_grad():
+        target = SP.grid_signal(SP.trimer_params(perturb=5), spec)
+    params = SP.trimer_params()
+    loss = SP.grid_loss(SP.grid_signal(params, spec), target)
+    loss.backward()
+    assert spec.timing["forward_adjoint"] is True
+    rng = np.random.default_rng(11)
+
+    def L_gpu(step):
+        p = {k: v.detach().clone() + step.get(k, 0.0) for k, v in params.items()}
+        with torch.no_grad():
+            return float(SP.grid_loss(SP.grid_signal(p, spec), target))
+
+    for names, rel in ((("eps", "J", "mu"), 1e-4), (("g", "lam"), 1e-3)):
+        d = {}
+        for k in names:
+            base = params[k].detach()
+            d[k] = torch.from_numpy(rng.standard_normal(tuple(base.shape))).to(base.dtype) * float(base.abs().max())
+        proj = sum(float(torch.sum(params[k].grad * d[k])) for k in names)
+
+        def central(h):
+            return (L_gpu({k: h * v for k, v in d.items()}) - L_gpu({k: -h * v for k, v in d.items()})) / (2 * h)
+
+        ref = (4 * central(rel / 2) - central(rel)) / 3
+        assert abs(proj) > 0 and abs(proj - ref) <= 1e-6 * abs(ref), (names, proj, ref)
