@@ -80,6 +80,10 @@ def test_n64_random_hermitian_system(ctx):
     p = strong(like(base, H=H.astype(complex), X=X, alpha=base.alpha[:3], alpha_t=base.alpha_t[:3],
                     gamma=base.gamma[:3]))
     assert p.N == 64
+    # a dense H has 64 entries a row: no sparse-commutator candidate
+    b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)}])
+    assert "stage_sparse" not in {k for _, _, k, _ in b.candidates()}
+    b.close()
     check(ctx, p)
 
 
