@@ -70,10 +70,11 @@ struct StageArgs {
     int fused;
     long long vstride, vpstride;
     const struct FusedDev* fz;   // what the fused tail reads (device memory: few kernel-argument registers)
-    // 1: the small-N stage's state stores (out, ACC) carry the non-temporal hint.  Measured:
-    // 3 % faster on a group whose state is far larger than the last-level cache (with the
-    // same L2 traffic and hit rate), 12 % slower on one that nearly fits it (the next stage
-    // reads its input back from that cache).  The MFMA kernels ignore it.
+    // 1: the small-N and sparse stages' state stores (out, ACC) carry the non-temporal hint.
+    // Measured: 3 % faster on a group whose state is far larger than the last-level cache
+    // (with the same L2 traffic and hit rate), 12 % slower on one that nearly fits it (the
+    // next stage reads its input back from that cache); the sparse stage on the 91 GB chain
+    // grid 1.6 % faster.  The MFMA kernels ignore it.
     int stream;
     // Sparsity pattern of V(t) over the whole run, per instance (row offset InstDev::sp_off):
     // row a holds sp_cnt[a] column indices sp_idx[a * kSpCap + r] (ascending, -1 beyond);

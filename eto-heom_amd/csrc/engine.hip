@@ -700,7 +700,7 @@ __global__ __launch_bounds__(512, 4) void stage_sparse(StageArgs A) {
             const double2 ww = wv[i];
             const double2 y0 = st == 0 ? x : ww;
             const double2 a1 = st == 1 ? x : (st == 3 ? make_double2(fma(x.x, 1.0 / 3.0, ww.x), fma(x.y, 1.0 / 3.0, ww.y)) : ww);
-            rk_store<false, true>(A, I, (size_t)(base + a * N + b), x, y0, a1, dr + cpl.x, di + cpl.y, a0, a, b);
+            rk_store<true, true>(A, I, (size_t)(base + a * N + b), x, y0, a1, dr + cpl.x, di + cpl.y, a0, a, b);
         }
     }
     if (A.fused && a0 == 0) fused_tail(A, I, step, tid, 512);
