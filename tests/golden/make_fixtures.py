@@ -40,6 +40,15 @@ def cases():
     h = configs.heptamer()
     h.t_end = h.t_start + h.step_size * 599
     out["heptamer_L4_600"] = (h, (1, 1, 1), "tiers")
+    # the 10-site chain of the benchmark grid (N = 56, L = 3, 1771 ADOs) with its full
+    # hierarchy: the three pulses moved to 4 / 10 / 16 fs (3 fs wide, field x1e4, so that
+    # every tier carries signal within the run), 60 steps: 45 with a pulse on (complex V)
+    # and 15 without, 28 readouts
+    c = configs.chain(nsteps=60)
+    c.pulses[:, 1] = [4.0, 10.0, 16.0]
+    c.pulses[:, 2] = 3.0
+    c.pulses[:, 0] *= 1e4
+    out["chain_L3_60"] = (c, (1, 1, 1), "tiers")
     return out
 
 

@@ -10,7 +10,9 @@
   3003 ADOs, 2000 steps) and the heptamer benchmark system with its full L = 4
   hierarchy (3060 ADOs) through all three pulse windows and its first 220
   readouts (600 steps); for the two larger ones the final rho_0, the first tier
-  and the norm of every ADO.
+  and the norm of every ADO; the 10-site chain of the benchmark grid with its full
+  L = 3 hierarchy (1771 ADOs, N = 56) over 60 steps with strong, early pulses, on every
+  launch candidate of its N class.
 
 At this amplitude rho_0 stays within O(E0^2) of |0><0|, so besides the usual
 normwise error over the whole state (which the 1 in rho_0[0][0] dominates) the
@@ -76,17 +78,8 @@ def load(name):
     return p, tuple(int(v) for v in f["nv"]), f
 
 
-@pytest.mark.parametrize("name,npoints,nado", [("trimer_L5_2000", 1620, 462), ("tetramer_L6_2000", 1620, 3003),
-                                               ("heptamer_L4_600", 220, 3060)])
-def test_whole_runs_match_oracle_fixtures(ctx, name, npoints, nado):
-    p, nv, f = load(name)
-    assert p.num_ado() == nado
-    b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": nv}])
-    b.run()
-    b.sync()
-    P, rho = b.output(0), b.rho(0)
-    b.close()
-    assert len(P) == len(f["P"]) == npoints
+def check_fixture(f, P, rho, name):
+    assert len(P) == len(f["P"])
     norms = np.sqrt(np.sum(np.abs(rho) ** 2, axis=(1, 2)))
     errs = {"P": rel(P, f["P"]), "ADO norms": rel(norms, f["ado_norms"]),
             "norms beyond rho_0": rel(norms[1:], f["ado_norms"][1:])}
@@ -100,3 +93,33 @@ def test_whole_runs_match_oracle_fixtures(ctx, name, npoints, nado):
     print(name, {k: f"{v:.2e}" for k, v in errs.items()})
     for k, v in errs.items():
         assert v < TOL, (k, v)
+
+
+@pytest.mark.parametrize("name,npoints,nado", [("trimer_L5_2000", 1620, 462), ("tetramer_L6_2000", 1620, 3003),
+                                               ("heptamer_L4_600", 220, 3060), ("chain_L3_60", 28, 1771)])
+def test_whole_runs_match_oracle_fixtures(ctx, name, npoints, nado):
+    p, nv, f = load(name)
+    assert p.num_ado() == nado
+    b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": nv}])
+    b.run()
+    b.sync()
+    P, rho = b.output(0), b.rho(0)
+    b.close()
+    assert len(P) == npoints
+    check_fixture(f, P, rho, name)
+
+
+def test_chain_full_hierarchy_every_launch_candidate(ctx):
+    """The chain fixture (N = 56, L = 3, 1771 ADOs, 45 steps with complex V then 15 with
+    V = H) on every launch candidate of its N class, the sparse-commutator default and the
+    dense MFMA kernels among them."""
+    p, nv, f = load("chain_L3_60")
+    b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": nv}])
+    cands = b.candidates()
+    assert {"stage_sparse_rows", "stage_sparse", "stage_mfma64"} <= {k for _, _, k, _ in cands}
+    for g, c, kern, _ in cands:
+        b.select(g, c)
+        b.run()
+        b.sync()
+        check_fixture(f, b.output(0), b.rho(0), f"chain_L3_60 {kern}")
+    b.close()
