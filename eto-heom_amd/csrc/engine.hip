@@ -706,6 +706,216 @@ __global__ __launch_bounds__(512, 4) void stage_sparse(StageArgs A) {
     if (A.fused && a0 == 0) fused_tail(A, I, step, tid, 512);
 }
 
+// the value of v in lane l (l a compile-time constant) as a wave-uniform scalar
+__device__ __forceinline__ double lane_d(double v, int l) {
+    const long long u = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(u >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// LDS layout of one tile of stage_sparse_rows (N and K of its instance)
+struct SxLds {
+    double2* sR;    // rho_n, N x (N + 1)
+    double2* sC;    // coupling rows/columns, 2 K N
+    double* sVr;    // [N][kSpCap + 1]: Re V[a][c] of row a's entries
+    short* sIx;     // [N][kSpCap + 1]: c (-1 beyond)
+};
+__device__ __forceinline__ SxLds sx_lds(double2* smem, int N, int K) {
+    SxLds L;
+    L.sR = smem;
+    L.sC = L.sR + N * (N + 1);
+    L.sVr = reinterpret_cast<double*>(L.sC + 2 * K * N);
+    L.sIx = reinterpret_cast<short*>(L.sVr + N * (kSpCap + 1));
+    return L;
+}
+
+// a tile's stage input and pattern lists into registers (every load unconditional: clamped
+// address, selected value); sx_fill writes them to LDS
+__device__ __forceinline__ void sx_load(const StageArgs& A, const InstDev& I, long long base, int tid, double2 (&xr)[8],
+                                        int (&ix)[2], double (&vc)[2]) {
+    constexpr int CAP = kSpCap;
+    const int N = I.N;
+    const double2* V = A.V + I.v_off;
+    const long long pbase = (long long)I.sp_off * CAP;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int e = tid + q * 512, r = e >> 6, c = e & 63;
+        const bool in = r < N && c < N;
+        const double2 t = A.rin[base + (in ? r * N + c : 0)];
+        xr[q] = in ? t : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) ix[h] = A.sp_idx[pbase + min(tid + h * 512, N * CAP - 1)];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int e = min(tid + h * 512, N * CAP - 1), a = e / CAP;
+        vc[h] = V[(long long)a * N + (ix[h] >= 0 ? ix[h] : 0)].x;   // Re V[a][c] of row a's entry
+    }
+}
+__device__ __forceinline__ void sx_fill(const InstDev& I, const SxLds& L, int tid, const double2 (&xr)[8],
+                                        const int (&ix)[2], const double (&vc)[2]) {
+    constexpr int CAP = kSpCap, PS = kSpCap + 1;
+    const int N = I.N, LD = N + 1;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int e = tid + q * 512, r = e >> 6, c = e & 63;
+        if (r < N && c < N) L.sR[r * LD + c] = xr[q];
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int e = tid + h * 512;
+        if (e < N * CAP) {
+            const int a = e / CAP, r = e - a * CAP;
+            L.sIx[a * PS + r] = (short)ix[h];
+            L.sVr[a * PS + r] = ix[h] >= 0 ? vc[h] : 0.0;
+        }
+    }
+}
+
+// The rows of one tile: the partial outputs p1 (out) without the (rho V) term, and
+// (rho V)[b][a] in tq.
+template <int S, int R>
+__device__ __forceinline__ void sx_rows(const StageArgs& A, const InstDev& I, int a0, long long base, const SxLds& L,
+                                        bool sym, double2 (&p1)[R], double2 (&tq)[R]) {
+    constexpr int CAP = kSpCap, PS = kSpCap + 1;
+    const int N = I.N, LD = N + 1;
+    const int tid = threadIdx.x, b = tid & 63, w = tid >> 6;
+    const int bc = b < N ? b : 0, l16 = b & (CAP - 1);
+    const double2* V = A.V + I.v_off;
+    // the state operand of the lane's elements (Y in stages 1-2, ACC in stage 3)
+    double2 wv[R];
+    if (S != 0) {
+        const double2* wp = S == 3 ? A.acc : A.y;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int a = w + 8 * i;
+            wv[i] = wp[base + (a < N && b < N ? a * N + b : 0)];
+        }
+    }
+    __syncthreads();   // the tile's input, lists and couplings are in LDS
+    const double2 g = ro(A.damp, I.ado_off + a0);
+    const double hc = S == 0 ? I.h2 : (S == 2 ? I.h : I.hb[3]);
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        const int a = w + 8 * i;
+        if (a >= N) break;   // uniform in the wave
+        // row a's entries, lane r holding entry r: its column c, V[a][c] (u) and V[c][a] (d)
+        // (u.x from the LDS list, read in both cases so that it stays an LDS read)
+        const int cr = L.sIx[a * PS + l16];
+        const double vs = L.sVr[a * PS + l16];
+        double2 u = make_double2(vs, 0.0), d = make_double2(0.0, 0.0);
+        if (!sym) {
+            const int cs = cr >= 0 ? cr : 0;
+            u = V[(long long)a * N + cs];
+            d = V[(long long)cs * N + a];
+        }
+        const int nra = __popcll(__ballot(b < CAP && cr >= 0));
+        double2 ar = make_double2(0.0, 0.0), at = ar;
+#pragma unroll 1
+        for (int r = 0; r < nra; ++r) {
+            const int c = __builtin_amdgcn_readlane(cr, r);
+            const double2 xa = L.sR[c * LD + bc], xb = L.sR[bc * LD + c];
+            if (sym) {
+                const double v = lane_d(u.x, r);
+                ar.x = fma(v, xa.x, ar.x);
+                ar.y = fma(v, xa.y, ar.y);
+                at.x = fma(xb.x, v, at.x);
+                at.y = fma(xb.y, v, at.y);
+            } else {
+                ar = cmad(ar, make_double2(lane_d(u.x, r), lane_d(u.y, r)), xa);
+                at = cmad(at, xb, make_double2(lane_d(d.x, r), lane_d(d.y, r)));
+            }
+        }
+        tq[i] = at;
+        // fold the damping, the coupling terms and the state operand in
+        const double2 x = L.sR[a * LD + bc];
+        double kr = ar.y, ki = -ar.x;   // -i * (V rho)
+        kr = fma(g.x, x.x, fma(-g.y, x.y, kr));
+        ki = fma(g.x, x.y, fma(g.y, x.x, ki));
+        const double2 cpl = coupling_of(I, L.sC, a, bc);
+        kr += cpl.x;
+        ki += cpl.y;
+        double2 o = S == 0 ? x : wv[i];   // out = o + hc k
+        if (S == 3) o = make_double2(fma(x.x, 1.0 / 3.0, o.x), fma(x.y, 1.0 / 3.0, o.y));
+        p1[i] = make_double2(fma(hc, kr, o.x), fma(hc, ki, o.y));
+    }
+}
+
+// Sparse commutator stage, row-pattern form (one instantiation per RK4 stage S and rows per
+// lane R).  Thread (lane b, row a) runs over row a's pattern P(a) (wave-uniform) once and
+// forms both
+//   (V rho)[a][b] = sum_{c in P(a)} V[a][c] rho[c][b]   (row c of rho: consecutive in the wave)
+//   (rho V)[b][a] = sum_{c in P(a)} rho[b][c] V[c][a]   (column c of rho: stride N + 1, odd,
+//                                                        so the wave's reads hit distinct banks)
+// (the pattern is symmetric, so P(a) is also column a's pattern).  Everything of the update
+// but the (rho V)[a][b] term goes into the element's partial outputs at once (the damping,
+// the coupling terms, the state operand); once every row is done the (rho V) values are
+// swapped through rho_n's LDS (element [b][a] to the thread that owns it) and completed.
+// Row a's entries (index, V values) sit one per lane in lanes 0..15 and reach the wave as
+// scalars by readlane; V real and symmetric: Re V[a][c] from the LDS list; otherwise
+// V[a][c] and V[c][a] are loaded from the step's V.  Summation order: each sum over P(a)
+// ascending (the exact zeros of the dense product are skipped; the sum differs from it by
+// rounding only).
+template <int S, int R>
+__global__ __launch_bounds__(512, 4) void stage_sparse_rows(StageArgs A) {
+    static_assert(S != 1, "stage 1 runs stage_sparse");
+    extern __shared__ double2 smem[];
+    const int step = *A.step_ctr + A.step_off;
+    at_step(A, step);
+    const int tile = blockIdx.x, tid = threadIdx.x, b = tid & 63, w = tid >> 6;
+    const InstDev I = ro(A.inst, ro(A.tile_inst, tile));   // scalar loads, then registers
+    if (step >= I.nsteps) return;
+    const int N = I.N, LD = N + 1;
+    const int a0 = ro(A.tile_ado0, tile);
+    const long long base = I.state_off + (long long)a0 * N * N;
+    const SxLds L = sx_lds(smem, N, I.K);
+    {
+        double2 xr[8];
+        int ix[2];
+        double vc[2];
+        sx_load(A, I, base, tid, xr, ix, vc);
+        gather_couplings(A, I, a0, 1, L.sC, tid, 512);
+        sx_fill(I, L, tid, xr, ix, vc);
+    }
+    double2 p1[R], tq[R];
+    sx_rows<S, R>(A, I, a0, base, L, A.vreal[I.step_off + step] != 0 && I.hsym, p1, tq);
+    // (rho V)[b][a] into rho_n's place [b][a] once every read of rho_n is done, then each
+    // thread's own (rho V)[a][b]: out = p1 + hc i (rho V)
+    __syncthreads();
+
+    if (b < N) {
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int a = w + 8 * i;
+            if (a < N) L.sR[b * LD + a] = tq[i];
+        }
+    }
+    __syncthreads();
+    const double hc = S == 0 ? I.h2 : (S == 2 ? I.h : I.hb[3]);
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        const int a = w + 8 * i;
+        if (a < N && b < N) {
+            const double2 t = L.sR[a * LD + b];
+            const size_t gi = (size_t)(base + a * N + b);
+            st_state<true>(A, A.out + gi, make_double2(fma(-hc, t.y, p1[i].x), fma(hc, t.x, p1[i].y)));
+        }
+    }
+    if (A.fused && a0 == 0) fused_tail(A, I, step, tid, 512);
+}
+
+// The kernel of stage `stage` in the row-pattern candidate: stage 1, which stores two
+// outputs (out and ACC), would hold a third array of partial values through the rows and
+// runs the column-list kernel instead
+typedef void (*StageKernel)(StageArgs);
+static StageKernel sparse_rows_kernel(int stage, int rows) {
+    static const StageKernel k[2][4] = {
+        {stage_sparse_rows<0, 7>, stage_sparse, stage_sparse_rows<2, 7>, stage_sparse_rows<3, 7>},
+        {stage_sparse_rows<0, 8>, stage_sparse, stage_sparse_rows<2, 8>, stage_sparse_rows<3, 8>}};
+    return k[rows == 7 ? 0 : 1][stage & 3];
+}
+
 // Coupling term of output element (row, col) of the ADO whose table row starts at lrow,
 // gathered straight from the stage input: the same element of the up / down neighbours
 // of the bath whose site is row (weights i x, i alpha y) and of the bath whose site is
@@ -1211,6 +1421,9 @@ int launch_cfg(const Group& g, const LaunchCfg& c, const StageArgs& base, hipStr
         case -65: hipLaunchKernelGGL(stage_mfma64, grid, dim3(512), c.shmem, s, a); break;
         case -128: hipLaunchKernelGGL(stage_mfma_wide, grid, block, 0, s, a); break;
         case -67: hipLaunchKernelGGL(stage_sparse, grid, dim3(512), c.shmem, s, a); break;
+        case -69:   // one kernel per RK4 stage and rows per lane (7: N <= 56, 8: N <= 64)
+            hipLaunchKernelGGL(sparse_rows_kernel(a.stage, g.maxnn <= 56 * 56 ? 7 : 8), grid, dim3(512), c.shmem, s, a);
+            break;
         case -33:
             // the RK coefficients of this stage are arguments (one time step per group)
             hipLaunchKernelGGL(stage_mfma_pipe<2>, dim3(std::min(c.tile_count, g.persist)), block, c.shmem, s, a,
@@ -1668,7 +1881,7 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
             }
             else if (nt == -128)
                 c.shmem = 0;   // stage_mfma_wide: nothing in LDS
-            else if (nt == -67) {   // stage_sparse: rho, the couplings, the column lists
+            else if (nt == -67 || nt == -69) {   // stage_sparse: rho, the couplings, the column lists
                 int mn = 1;
                 while (mn * mn < maxnn) mn++;
                 const size_t ps = kSpCap + 1;
@@ -1684,9 +1897,14 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
                 const void* fn = nt == -64   ? (const void*)stage_mfma<64, false>
                                  : nt == -65 ? (const void*)stage_mfma64
                                  : nt == -67 ? (const void*)stage_sparse
+                                 : nt == -69 ? (const void*)sparse_rows_kernel(0, maxnn <= 56 * 56 ? 7 : 8)
                                  : (nt == -33 ? (const void*)stage_mfma_pipe<2> : (const void*)stage_generic);
                 if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.shmem) != hipSuccess)
                     return;
+                for (int st = 0; nt == -69 && st < 4; st++)   // one kernel per RK4 stage
+                    if (hipFuncSetAttribute((const void*)sparse_rows_kernel(st, maxnn <= 56 * 56 ? 7 : 8),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.shmem) != hipSuccess)
+                        return;
             }
             g.cand.push_back(c);
         };
@@ -1710,7 +1928,8 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
         } else {   // 32 < N <= 64: stage_mfma64 gathers columns directly (measured faster at N = 56)
             if (use_mfma) add(-65, 0, false, stream);
             add(0, 0, true, stream);
-            if (sparse_ok) add(-67, 0, false, stream);   // the default (below)
+            if (sparse_ok) add(-67, 0, false, stream);
+            if (sparse_ok) add(-69, 0, false, stream);   // the default (below)
             if (use_mfma) {
                 add(-65, 0, true, stream);
                 add(-64, 0, true, stream);
@@ -1726,7 +1945,7 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
             return fail(EH_ESHAPE, "state tile exceeds LDS");
         for (size_t c = 0; c < g.cand.size(); c++) {
             if (use_pipe && g.cand[c].nt == -33) g.sel = (int)c;
-            if (g.cand[c].nt == -67) g.sel = (int)c;   // measured: chain grid 25.5 -> 20.5 ms per stage
+            if (g.cand[c].nt == -69) g.sel = (int)c;   // measured: chain grid 78.2 -> 73.9 ms per RK4 step
         }
         g.nt = g.cand[0].nt;
         g.tile_begin = g.cand[0].tile_begin;
@@ -2055,6 +2274,7 @@ std::string batch_tuning_report(const Batch* b) {
                              : x.nt == -64 ? "stage_mfma64_v1"
                              : x.nt == -65 ? "stage_mfma64"
                              : x.nt == -67 ? "stage_sparse"
+                             : x.nt == -69 ? "stage_sparse_rows"
                              : x.nt == -33 ? "stage_mfma_pipe"
                              : x.nt == -128 ? "stage_mfma_wide"
                                             : "stage_generic";

@@ -86,7 +86,8 @@ def test_every_launch_candidate_matches_oracle(ctx, refs):
     cands = b.candidates()
     kinds = {k for _, _, k, _ in cands}
     # the N = 29 item also admits the pipelined and two-phase kernels (K_m = 2, 2 K N <= 512)
-    mfma = ({"stage_mfma32", "stage_mfma32_tp", "stage_mfma_pipe", "stage_mfma64", "stage_sparse"}
+    mfma = ({"stage_mfma32", "stage_mfma32_tp", "stage_mfma_pipe", "stage_mfma64", "stage_sparse",
+             "stage_sparse_rows"}
             if os.environ.get("ETOHEOM_NO_MFMA") != "1" else set())
     assert {"stage_small", "stage_generic"} | mfma <= kinds
     assert {t for _, _, k, t in cands if k == "stage_small"} == {256, 512, 1024}

@@ -156,11 +156,12 @@ def test_chain_n56_every_launch_candidate_matches_oracle(ctx):
     b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)} for p in ps])
     cands = b.candidates()
     assert len(cands) >= 3
-    # V's pattern has at most 13 entries a row: the sparse kernel is offered and is the
-    # default; stage_mfma64 stays the base (candidate 0) whose tiling the adjoint follows
+    # V's pattern has at most 13 entries a row: the sparse kernels are offered and the
+    # row-pattern one is the default; stage_mfma64 stays the base (candidate 0) whose tiling
+    # the adjoint follows
     rep = b.tuning_report().splitlines()
     assert [k for _, _, k, _ in cands][0] == "stage_mfma64"
-    assert any("stage_sparse" in l and l.endswith("*") for l in rep), rep
+    assert any(" stage_sparse_rows " in l and l.endswith("*") for l in rep), rep
     for g, c, kern, _ in cands:
         b.select(g, c)
         b.run()
