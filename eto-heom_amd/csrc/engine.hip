@@ -777,15 +777,15 @@ __device__ __forceinline__ void sx_fill(const InstDev& I, const SxLds& L, int ti
 // (rho V)[b][a] in tq.
 template <int S, int R>
 __device__ __forceinline__ void sx_rows(const StageArgs& A, const InstDev& I, int a0, long long base, const SxLds& L,
-                                        bool sym, double2 (&p1)[R], double2 (&tq)[R]) {
+                                        bool sym, double2 (&wv)[R], double2 (&p1)[R], double2 (&tq)[R]) {
     constexpr int CAP = kSpCap, PS = kSpCap + 1;
     const int N = I.N, LD = N + 1;
     const int tid = threadIdx.x, b = tid & 63, w = tid >> 6;
     const int bc = b < N ? b : 0, l16 = b & (CAP - 1);
     const double2* V = A.V + I.v_off;
-    // the state operand of the lane's elements (Y in stages 1-2, ACC in stage 3)
-    double2 wv[R];
-    if (S != 0) {
+    // the state operand of the lane's elements (Y in stage 2, ACC in stage 3; stage 1 loads Y
+    // after the rows)
+    if (S >= 2) {
         const double2* wp = S == 3 ? A.acc : A.y;
 #pragma unroll
         for (int i = 0; i < R; ++i) {
@@ -795,7 +795,7 @@ __device__ __forceinline__ void sx_rows(const StageArgs& A, const InstDev& I, in
     }
     __syncthreads();   // the tile's input, lists and couplings are in LDS
     const double2 g = ro(A.damp, I.ado_off + a0);
-    const double hc = S == 0 ? I.h2 : (S == 2 ? I.h : I.hb[3]);
+    const double hc = S == 0 || S == 1 ? I.h2 : (S == 2 ? I.h : I.hb[3]);
 #pragma unroll
     for (int i = 0; i < R; ++i) {
         const int a = w + 8 * i;
@@ -836,9 +836,13 @@ __device__ __forceinline__ void sx_rows(const StageArgs& A, const InstDev& I, in
         const double2 cpl = coupling_of(I, L.sC, a, bc);
         kr += cpl.x;
         ki += cpl.y;
-        double2 o = S == 0 ? x : wv[i];   // out = o + hc k
-        if (S == 3) o = make_double2(fma(x.x, 1.0 / 3.0, o.x), fma(x.y, 1.0 / 3.0, o.y));
-        p1[i] = make_double2(fma(hc, kr, o.x), fma(hc, ki, o.y));
+        if (S == 1) {   // stage 1 keeps the partial derivative: its two outputs follow the exchange
+            p1[i] = make_double2(kr, ki);
+        } else {
+            double2 o = S == 0 ? x : wv[i];   // out = o + hc k
+            if (S == 3) o = make_double2(fma(x.x, 1.0 / 3.0, o.x), fma(x.y, 1.0 / 3.0, o.y));
+            p1[i] = make_double2(fma(hc, kr, o.x), fma(hc, ki, o.y));
+        }
     }
 }
 
@@ -859,7 +863,6 @@ __device__ __forceinline__ void sx_rows(const StageArgs& A, const InstDev& I, in
 // rounding only).
 template <int S, int R>
 __global__ __launch_bounds__(512, 4) void stage_sparse_rows(StageArgs A) {
-    static_assert(S != 1, "stage 1 runs stage_sparse");
     extern __shared__ double2 smem[];
     const int step = *A.step_ctr + A.step_off;
     at_step(A, step);
@@ -878,12 +881,50 @@ __global__ __launch_bounds__(512, 4) void stage_sparse_rows(StageArgs A) {
         gather_couplings(A, I, a0, 1, L.sC, tid, 512);
         sx_fill(I, L, tid, xr, ix, vc);
     }
-    double2 p1[R], tq[R];
-    sx_rows<S, R>(A, I, a0, base, L, A.vreal[I.step_off + step] != 0 && I.hsym, p1, tq);
-    // (rho V)[b][a] into rho_n's place [b][a] once every read of rho_n is done, then each
-    // thread's own (rho V)[a][b]: out = p1 + hc i (rho V)
-    __syncthreads();
-
+    double2 wv[R], p1[R], tq[R];
+    sx_rows<S, R>(A, I, a0, base, L, A.vreal[I.step_off + step] != 0 && I.hsym, wv, p1, tq);
+    __syncthreads();   // every read of rho_n, the couplings and the lists is done
+    if (S == 1) {
+        // two outputs, out = Y + h2 k and ACC = (x + Y) / 3 + hb[1] k, from k = k' + i (rho V):
+        // x stays in rho_n's LDS, and the (rho V) values go through the LDS of the couplings
+        // and lists in two rounds of H rows (the launch holds H (N | 1) elements there)
+        const int H = (N + 1) / 2, LE = N | 1;
+        double2* sE = L.sC;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int a = w + 8 * i;
+            wv[i] = A.y[base + (a < N && b < N ? a * N + b : 0)];
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (h) __syncthreads();   // the first round's values are read
+            if (b >= h * H && b < N && b < (h + 1) * H) {
+#pragma unroll
+                for (int i = 0; i < R; ++i) {
+                    const int a = w + 8 * i;
+                    if (a < N) sE[(b - h * H) * LE + a] = tq[i];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                const int a = w + 8 * i;
+                if (a >= h * H && a < N && a < (h + 1) * H && b < N) {   // a: uniform in the wave
+                    const double2 t = sE[(a - h * H) * LE + b];   // (rho V)[a][b]
+                    const double2 x = L.sR[a * LD + b], y = wv[i];
+                    const double kr = p1[i].x - t.y, ki = p1[i].y + t.x;   // k' + i (rho V)
+                    const size_t gi = (size_t)(base + a * N + b);
+                    const double2 q = make_double2((x.x + y.x) * (1.0 / 3.0), (x.y + y.y) * (1.0 / 3.0));
+                    st_state<true>(A, A.acc + gi, make_double2(fma(I.hb[1], kr, q.x), fma(I.hb[1], ki, q.y)));
+                    st_state<true>(A, A.out + gi, make_double2(fma(I.h2, kr, y.x), fma(I.h2, ki, y.y)));
+                }
+            }
+        }
+        if (A.fused && a0 == 0) fused_tail(A, I, step, tid, 512);
+        return;
+    }
+    // (rho V)[b][a] into rho_n's place [b][a], then each thread's own (rho V)[a][b]:
+    // out = p1 + hc i (rho V)
     if (b < N) {
 #pragma unroll
         for (int i = 0; i < R; ++i) {
@@ -905,14 +946,15 @@ __global__ __launch_bounds__(512, 4) void stage_sparse_rows(StageArgs A) {
     if (A.fused && a0 == 0) fused_tail(A, I, step, tid, 512);
 }
 
-// The kernel of stage `stage` in the row-pattern candidate: stage 1, which stores two
-// outputs (out and ACC), would hold a third array of partial values through the rows and
-// runs the column-list kernel instead
+// The kernel of stage `stage` in the row-pattern candidate.  Stage 1 stores two outputs
+// (out and ACC) and exchanges its (rho V) values through the LDS past rho_n, in two rounds;
+// where that LDS is too small for half of them (rows1 false) it runs the column-list kernel.
 typedef void (*StageKernel)(StageArgs);
-static StageKernel sparse_rows_kernel(int stage, int rows) {
+static StageKernel sparse_rows_kernel(int stage, int rows, bool rows1) {
     static const StageKernel k[2][4] = {
-        {stage_sparse_rows<0, 7>, stage_sparse, stage_sparse_rows<2, 7>, stage_sparse_rows<3, 7>},
-        {stage_sparse_rows<0, 8>, stage_sparse, stage_sparse_rows<2, 8>, stage_sparse_rows<3, 8>}};
+        {stage_sparse_rows<0, 7>, stage_sparse_rows<1, 7>, stage_sparse_rows<2, 7>, stage_sparse_rows<3, 7>},
+        {stage_sparse_rows<0, 8>, stage_sparse_rows<1, 8>, stage_sparse_rows<2, 8>, stage_sparse_rows<3, 8>}};
+    if ((stage & 3) == 1 && !rows1) return stage_sparse;
     return k[rows == 7 ? 0 : 1][stage & 3];
 }
 
@@ -1422,7 +1464,8 @@ int launch_cfg(const Group& g, const LaunchCfg& c, const StageArgs& base, hipStr
         case -128: hipLaunchKernelGGL(stage_mfma_wide, grid, block, 0, s, a); break;
         case -67: hipLaunchKernelGGL(stage_sparse, grid, dim3(512), c.shmem, s, a); break;
         case -69:   // one kernel per RK4 stage and rows per lane (7: N <= 56, 8: N <= 64)
-            hipLaunchKernelGGL(sparse_rows_kernel(a.stage, g.maxnn <= 56 * 56 ? 7 : 8), grid, dim3(512), c.shmem, s, a);
+            hipLaunchKernelGGL(sparse_rows_kernel(a.stage, g.maxnn <= 56 * 56 ? 7 : 8, c.rows1), grid, dim3(512),
+                               c.shmem, s, a);
             break;
         case -33:
             // the RK coefficients of this stage are arguments (one time step per group)
@@ -1887,6 +1930,11 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
                 const size_t ps = kSpCap + 1;
                 c.shmem = (size_t)mn * (mn + 1) * sizeof(double2) + cpb + (size_t)mn * ps * sizeof(double) +
                           (((size_t)mn * ps * sizeof(short) + 3) & ~(size_t)3) + (size_t)mn * sizeof(int);
+                // stage 1 of stage_sparse_rows exchanges half of rho V's rows at a time past rho
+                // (and holds one array more: seven rows per lane, N <= 56, fit the registers)
+                c.rows1 = nt == -69 && mn <= 56 &&
+                          (size_t)mn * (mn + 1) * sizeof(double2) +
+                                  (size_t)((mn + 1) / 2) * (mn | 1) * sizeof(double2) <= c.shmem;
             }
             else if (nt > 0)
                 c.shmem = (size_t)G * cpb;   // the coupling rows/columns of G ADOs (the rest is static)
@@ -1897,12 +1945,12 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
                 const void* fn = nt == -64   ? (const void*)stage_mfma<64, false>
                                  : nt == -65 ? (const void*)stage_mfma64
                                  : nt == -67 ? (const void*)stage_sparse
-                                 : nt == -69 ? (const void*)sparse_rows_kernel(0, maxnn <= 56 * 56 ? 7 : 8)
+                                 : nt == -69 ? (const void*)sparse_rows_kernel(0, maxnn <= 56 * 56 ? 7 : 8, false)
                                  : (nt == -33 ? (const void*)stage_mfma_pipe<2> : (const void*)stage_generic);
                 if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.shmem) != hipSuccess)
                     return;
                 for (int st = 0; nt == -69 && st < 4; st++)   // one kernel per RK4 stage
-                    if (hipFuncSetAttribute((const void*)sparse_rows_kernel(st, maxnn <= 56 * 56 ? 7 : 8),
+                    if (hipFuncSetAttribute((const void*)sparse_rows_kernel(st, maxnn <= 56 * 56 ? 7 : 8, c.rows1),
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.shmem) != hipSuccess)
                         return;
             }

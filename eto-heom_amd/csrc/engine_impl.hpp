@@ -54,11 +54,13 @@ struct DevBuf {
 struct LaunchCfg {
     int nt;                   // 1..16: stage_small<nt>; -32 / -64: stage_mfma; -65: stage_mfma64;
                               // -33: stage_mfma_pipe; -128: stage_mfma_wide; -67: stage_sparse;
-                              // -69: stage_sparse_rows (stage 1: stage_sparse); 0: stage_generic
+                              // -69: stage_sparse_rows (stage 1: stage_sparse unless rows1);
+                              // 0: stage_generic
     int ge;                   // stage_small: state elements per tile (256, 512 or 1024)
     bool cols;                // keep coupled-column copies (contiguous column gathers)
     bool stream;              // non-temporal state stores (StageArgs::stream)
     bool late = false;        // stage_small<N, 512, true>: state operand loaded after the barrier
+    bool rows1 = false;       // stage_sparse_rows: stage 1 too (its exchange fits the LDS past rho)
     int tile_begin, tile_count;
     size_t shmem;
 };
