@@ -800,29 +800,30 @@ __device__ __forceinline__ void sx_rows(const StageArgs& A, const InstDev& I, in
     for (int i = 0; i < R; ++i) {
         const int a = w + 8 * i;
         if (a >= N) break;   // uniform in the wave
-        // row a's entries, lane r holding entry r: its column c, V[a][c] (u) and V[c][a] (d)
-        // (u.x from the LDS list, read in both cases so that it stays an LDS read)
+        // row a's entries, lane r holding entry r: its column c and Re V[a][c] (vs, from the
+        // LDS list), or V[a][c] (u) and V[c][a] (d) from the step's V
         const int cr = L.sIx[a * PS + l16];
-        const double vs = L.sVr[a * PS + l16];
-        double2 u = make_double2(vs, 0.0), d = make_double2(0.0, 0.0);
-        if (!sym) {
-            const int cs = cr >= 0 ? cr : 0;
-            u = V[(long long)a * N + cs];
-            d = V[(long long)cs * N + a];
-        }
         const int nra = __popcll(__ballot(b < CAP && cr >= 0));
         double2 ar = make_double2(0.0, 0.0), at = ar;
+        if (sym) {
+            const double vs = L.sVr[a * PS + l16];
 #pragma unroll 1
-        for (int r = 0; r < nra; ++r) {
-            const int c = __builtin_amdgcn_readlane(cr, r);
-            const double2 xa = L.sR[c * LD + bc], xb = L.sR[bc * LD + c];
-            if (sym) {
-                const double v = lane_d(u.x, r);
+            for (int r = 0; r < nra; ++r) {
+                const int c = __builtin_amdgcn_readlane(cr, r);
+                const double2 xa = L.sR[c * LD + bc], xb = L.sR[bc * LD + c];
+                const double v = lane_d(vs, r);
                 ar.x = fma(v, xa.x, ar.x);
                 ar.y = fma(v, xa.y, ar.y);
                 at.x = fma(xb.x, v, at.x);
                 at.y = fma(xb.y, v, at.y);
-            } else {
+            }
+        } else {
+            const int cs = cr >= 0 ? cr : 0;
+            const double2 u = V[(long long)a * N + cs], d = V[(long long)cs * N + a];
+#pragma unroll 1
+            for (int r = 0; r < nra; ++r) {
+                const int c = __builtin_amdgcn_readlane(cr, r);
+                const double2 xa = L.sR[c * LD + bc], xb = L.sR[bc * LD + c];
                 ar = cmad(ar, make_double2(lane_d(u.x, r), lane_d(u.y, r)), xa);
                 at = cmad(at, xb, make_double2(lane_d(d.x, r), lane_d(d.y, r)));
             }
