@@ -732,14 +732,15 @@ __device__ __forceinline__ SxLds sx_lds(double2* smem, int N, int K) {
 
 // a tile's stage input and pattern lists into registers (every load unconditional: clamped
 // address, selected value); sx_fill writes them to LDS
-__device__ __forceinline__ void sx_load(const StageArgs& A, const InstDev& I, long long base, int tid, double2 (&xr)[8],
+template <int R>
+__device__ __forceinline__ void sx_load(const StageArgs& A, const InstDev& I, long long base, int tid, double2 (&xr)[R],
                                         int (&ix)[2], double (&vc)[2]) {
     constexpr int CAP = kSpCap;
     const int N = I.N;
     const double2* V = A.V + I.v_off;
     const long long pbase = (long long)I.sp_off * CAP;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < R; ++q) {   // rows 8q .. 8q + 7 (N <= 8R)
         const int e = tid + q * 512, r = e >> 6, c = e & 63;
         const bool in = r < N && c < N;
         const double2 t = A.rin[base + (in ? r * N + c : 0)];
@@ -753,12 +754,13 @@ __device__ __forceinline__ void sx_load(const StageArgs& A, const InstDev& I, lo
         vc[h] = V[(long long)a * N + (ix[h] >= 0 ? ix[h] : 0)].x;   // Re V[a][c] of row a's entry
     }
 }
-__device__ __forceinline__ void sx_fill(const InstDev& I, const SxLds& L, int tid, const double2 (&xr)[8],
+template <int R>
+__device__ __forceinline__ void sx_fill(const InstDev& I, const SxLds& L, int tid, const double2 (&xr)[R],
                                         const int (&ix)[2], const double (&vc)[2]) {
     constexpr int CAP = kSpCap, PS = kSpCap + 1;
     const int N = I.N, LD = N + 1;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < R; ++q) {
         const int e = tid + q * 512, r = e >> 6, c = e & 63;
         if (r < N && c < N) L.sR[r * LD + c] = xr[q];
     }
@@ -874,13 +876,28 @@ __global__ __launch_bounds__(512, 4) void stage_sparse_rows(StageArgs A) {
     const int a0 = ro(A.tile_ado0, tile);
     const long long base = I.state_off + (long long)a0 * N * N;
     const SxLds L = sx_lds(smem, N, I.K);
-    {
-        double2 xr[8];
+    if (S != 1) {
+        // the input, the lists and the coupling gathers' tables in flight together; the
+        // input into LDS (its registers free), then the neighbour rows/columns (two-phase
+        // gathers: three entries and two modes per thread, any more by gather_rest).
+        // Measured on the chain grid: stage 0 14.1 -> 13.3 ms, stages 2-3 unchanged; stage 1,
+        // whose rows hold more registers, spills with it and keeps the one-phase gathers.
+        double2 xr[R];
         int ix[2];
         double vc[2];
-        sx_load(A, I, base, tid, xr, ix, vc);
+        NbIdx<3, 2> nbx;
+        sx_load<R>(A, I, base, tid, xr, ix, vc);
+        gather_idx<3, 2>(A, I, a0, 1, tid, 512, nbx);
+        sx_fill<R>(I, L, tid, xr, ix, vc);
+        gather_data<3, 2>(A, I, 1, tid, 512, nbx, L.sC);
+        if (2 * I.K * N > 3 * 512 || I.Km > 2) gather_rest<3, 2>(A, I, a0, 1, tid, 512, L.sC);
+    } else {
+        double2 xr[R];
+        int ix[2];
+        double vc[2];
+        sx_load<R>(A, I, base, tid, xr, ix, vc);
         gather_couplings(A, I, a0, 1, L.sC, tid, 512);
-        sx_fill(I, L, tid, xr, ix, vc);
+        sx_fill<R>(I, L, tid, xr, ix, vc);
     }
     double2 wv[R], p1[R], tq[R];
     sx_rows<S, R>(A, I, a0, base, L, A.vreal[I.step_off + step] != 0 && I.hsym, wv, p1, tq);
@@ -958,6 +975,8 @@ static StageKernel sparse_rows_kernel(int stage, int rows, bool rows1) {
     if ((stage & 3) == 1 && !rows1) return stage_sparse;
     return k[rows == 7 ? 0 : 1][stage & 3];
 }
+// rows per lane of the row-pattern kernel for a group (32 < N <= 64) whose largest N^2 is maxnn
+static int sparse_rows_of(int maxnn) { return maxnn <= 56 * 56 ? 7 : 8; }
 
 // Coupling term of output element (row, col) of the ADO whose table row starts at lrow,
 // gathered straight from the stage input: the same element of the up / down neighbours
@@ -1465,7 +1484,7 @@ int launch_cfg(const Group& g, const LaunchCfg& c, const StageArgs& base, hipStr
         case -128: hipLaunchKernelGGL(stage_mfma_wide, grid, block, 0, s, a); break;
         case -67: hipLaunchKernelGGL(stage_sparse, grid, dim3(512), c.shmem, s, a); break;
         case -69:   // one kernel per RK4 stage and rows per lane (7: N <= 56, 8: N <= 64)
-            hipLaunchKernelGGL(sparse_rows_kernel(a.stage, g.maxnn <= 56 * 56 ? 7 : 8, c.rows1), grid, dim3(512),
+            hipLaunchKernelGGL(sparse_rows_kernel(a.stage, sparse_rows_of(g.maxnn), c.rows1), grid, dim3(512),
                                c.shmem, s, a);
             break;
         case -33:
@@ -1946,12 +1965,12 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
                 const void* fn = nt == -64   ? (const void*)stage_mfma<64, false>
                                  : nt == -65 ? (const void*)stage_mfma64
                                  : nt == -67 ? (const void*)stage_sparse
-                                 : nt == -69 ? (const void*)sparse_rows_kernel(0, maxnn <= 56 * 56 ? 7 : 8, false)
+                                 : nt == -69 ? (const void*)sparse_rows_kernel(0, sparse_rows_of(maxnn), false)
                                  : (nt == -33 ? (const void*)stage_mfma_pipe<2> : (const void*)stage_generic);
                 if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.shmem) != hipSuccess)
                     return;
                 for (int st = 0; nt == -69 && st < 4; st++)   // one kernel per RK4 stage
-                    if (hipFuncSetAttribute((const void*)sparse_rows_kernel(st, maxnn <= 56 * 56 ? 7 : 8, c.rows1),
+                    if (hipFuncSetAttribute((const void*)sparse_rows_kernel(st, sparse_rows_of(maxnn), c.rows1),
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.shmem) != hipSuccess)
                         return;
             }
