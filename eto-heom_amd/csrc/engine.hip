@@ -718,7 +718,8 @@ __device__ __forceinline__ double lane_d(double v, int l) {
 struct SxLds {
     double2* sR;    // rho_n, N x (N + 1)
     double2* sC;    // coupling rows/columns, 2 K N
-    double* sVr;    // [N][kSpCap + 1]: Re V[a][c] of row a's entries
+    double* sVr;    // [N][kSpCap + 1]: Re V[a][c] of row a's entries (complex V: the waves' rows'
+                    // V[a][c] and V[c][a], 2 kSpCap complex values per wave)
     short* sIx;     // [N][kSpCap + 1]: c (-1 beyond)
 };
 __device__ __forceinline__ SxLds sx_lds(double2* smem, int N, int K) {
@@ -820,14 +821,21 @@ __device__ __forceinline__ void sx_rows(const StageArgs& A, const InstDev& I, in
                 at.y = fma(xb.y, v, at.y);
             }
         } else {
+            // V[a][c] and V[c][a] of the row's entries into this wave's slots (the Re V list
+            // is not read with complex V), then read back as LDS broadcasts
             const int cs = cr >= 0 ? cr : 0;
             const double2 u = V[(long long)a * N + cs], d = V[(long long)cs * N + a];
+            double2* slot = reinterpret_cast<double2*>(L.sVr) + w * 2 * CAP;
+            if (b < CAP) {
+                slot[b] = u;
+                slot[CAP + b] = d;
+            }
 #pragma unroll 1
             for (int r = 0; r < nra; ++r) {
                 const int c = __builtin_amdgcn_readlane(cr, r);
                 const double2 xa = L.sR[c * LD + bc], xb = L.sR[bc * LD + c];
-                ar = cmad(ar, make_double2(lane_d(u.x, r), lane_d(u.y, r)), xa);
-                at = cmad(at, xb, make_double2(lane_d(d.x, r), lane_d(d.y, r)));
+                ar = cmad(ar, slot[r], xa);
+                at = cmad(at, xb, slot[CAP + r]);
             }
         }
         tq[i] = at;
