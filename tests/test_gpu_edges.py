@@ -2,7 +2,8 @@
 normwise relative, FP64):
 
 * N = 64, the largest system (no padding in the 64 x 64 MFMA tiles), with a random
-  Hermitian H and random dipoles;
+  Hermitian H and random dipoles; N = 60 with a banded H and sparse dipoles on every
+  launch candidate (the sparse kernels at eight rows per lane);
 * L = 0: the hierarchy is rho_0 alone (no neighbour links);
 * K = 0: no bath at all (d = 0, one ADO), including N = 1;
 * instances of different lengths and N classes in one batch, each read out to its own
@@ -85,6 +86,43 @@ def test_n64_random_hermitian_system(ctx):
     assert not {"stage_sparse", "stage_sparse_rows"} & {k for _, _, k, _ in b.candidates()}
     b.close()
     check(ctx, p)
+
+
+def test_n60_banded_system_every_candidate(ctx):
+    """N = 60 (eight rows per lane in the row-pattern sparse kernel, whose stage 1 then runs
+    the column-list kernel): a banded random H (|a - c| <= 3) and dipoles from the ground
+    state to ten states, so V has at most 11 entries a row; three baths, L = 1, 40 steps with
+    the pulses' complex V first, then real V.  Every launch candidate against the oracle."""
+    base = configs.chain(L=1, nsteps=40)
+    rng = np.random.default_rng(60)
+    N = 60
+    H = np.diag(rng.uniform(2.2, 2.45, N))
+    for k in range(1, 4):
+        v = rng.normal(size=N - k) * 0.02
+        H += np.diag(v, k) + np.diag(v, -k)
+    H[0, :] = 0.0
+    H[:, 0] = 0.0
+    X = np.zeros((4, N, N), complex)
+    for q in range(4):
+        v = rng.normal(size=10)
+        X[q, 0, 1:11] = v
+        X[q, 1:11, 0] = v
+    p = strong(like(base, H=H.astype(complex), X=X, alpha=base.alpha[:3], alpha_t=base.alpha_t[:3],
+                    gamma=base.gamma[:3]))
+    assert p.N == 60
+    b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)}])
+    cands = b.candidates()
+    assert {"stage_sparse", "stage_sparse_rows"} <= {k for _, _, k, _ in cands}
+    Pr, rr = O.twod(oproblem(p), nv=(1, 1, 1), want_rho=True)
+    assert np.max(np.abs(rr[1:])) > 1e-12 * np.max(np.abs(rr[0]))
+    for g, c, kern, _ in cands:
+        print(f"N=60 candidate {c}: {kern}", flush=True)
+        b.select(g, c)
+        b.run()
+        b.sync()
+        assert rel(b.output(0), Pr) < TOL, kern
+        assert rel(b.rho(0), rr) < TOL, kern
+    b.close()
 
 
 def test_depth_zero_hierarchy(ctx):
