@@ -830,6 +830,9 @@ __device__ __forceinline__ void sx_rows(const StageArgs& A, const InstDev& I, in
                 slot[b] = u;
                 slot[CAP + b] = d;
             }
+            // the wave reads what its own lanes wrote: a wave's LDS operations complete in
+            // issue order, so no barrier; this keeps the compiler from moving them
+            __builtin_amdgcn_wave_barrier();
 #pragma unroll 1
             for (int r = 0; r < nra; ++r) {
                 const int c = __builtin_amdgcn_readlane(cr, r);
@@ -867,11 +870,12 @@ __device__ __forceinline__ void sx_rows(const StageArgs& A, const InstDev& I, in
 // but the (rho V)[a][b] term goes into the element's partial outputs at once (the damping,
 // the coupling terms, the state operand); once every row is done the (rho V) values are
 // swapped through rho_n's LDS (element [b][a] to the thread that owns it) and completed.
-// Row a's entries (index, V values) sit one per lane in lanes 0..15 and reach the wave as
-// scalars by readlane; V real and symmetric: Re V[a][c] from the LDS list; otherwise
-// V[a][c] and V[c][a] are loaded from the step's V.  Summation order: each sum over P(a)
-// ascending (the exact zeros of the dense product are skipped; the sum differs from it by
-// rounding only).
+// Row a's entries sit one per lane in lanes 0..15: the column c reaches the wave by
+// readlane.  V real and symmetric: Re V[a][c] from the LDS list, also by readlane.
+// Otherwise V[a][c] and V[c][a] are loaded from the step's V, written to the wave's slots
+// in the LDS of that list (not read then), and read back as LDS broadcasts.  Summation
+// order: each sum over P(a) ascending (the exact zeros of the dense product are skipped; the
+// sum differs from it by rounding only).
 template <int S, int R>
 __global__ __launch_bounds__(512, 4) void stage_sparse_rows(StageArgs A) {
     extern __shared__ double2 smem[];
