@@ -1910,11 +1910,22 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
         g.h2 = f0.h2;
         for (int k = 0; k < 4; k++) g.hb[k] = f0.hb[k];
         std::vector<std::pair<int, std::pair<int, int>>> ranges;   // ADOs per tile -> (begin, count)
+        // with the XCD order below, the instances go to the slices round-robin (member k to
+        // slice k mod xcd): neighbouring grid points, which have their pulses on at the same
+        // steps, then load every XCD alike instead of a few (measured on the chain grid:
+        // RK4 steps 250-1249 in 50-step replays 3.35-3.57 s -> 3.29-3.36 s, where the steps
+        // without any pulse take 3.25 s)
+        std::vector<int> order(members.begin(), members.end());
+        if (xcd > 1 && cls > 16) {
+            order.clear();
+            for (int x = 0; x < xcd; x++)
+                for (size_t k = (size_t)x; k < members.size(); k += (size_t)xcd) order.push_back(members[k]);
+        }
         auto tiles = [&](int G) {
             for (const auto& r : ranges)
                 if (r.first == G) return r.second;
             const int begin = (int)t_inst.size();
-            for (int i : members) {
+            for (int i : order) {
                 const int M = hinst[(size_t)i].M;
                 for (int a0 = 0; a0 < M; a0 += G) {
                     t_inst.push_back(i);
