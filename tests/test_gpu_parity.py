@@ -173,6 +173,35 @@ def test_chain_n56_every_launch_candidate_matches_oracle(ctx):
     b.close()
 
 
+def test_mixed_sizes_in_the_sparse_group_every_candidate(ctx):
+    """One 32 < N <= 64 group holding N = 37, 46 and 56 (8- and 9-site aggregates, the chain;
+    L = 1, 40 steps through three strong pulses): every instance lays its rho, couplings and
+    pattern lists out in LDS at its own N and K inside the group's allocation.  Every launch
+    candidate of the group against the oracle."""
+    ps = [configs.aggregate(8, 1, nsteps=40, taus=(2.0, 5.0, 8.0)),
+          configs.aggregate(9, 1, nsteps=40, taus=(3.0, 6.0, 9.0)),
+          configs.chain(L=1, nsteps=40)]
+    ps[2].pulses[:, 1] = [2.5, 5.5, 8.5]
+    for p in ps:
+        p.pulses[:, 2] = 3.0
+        p.pulses[:, 0] *= 1e5
+    assert [p.N for p in ps] == [37, 46, 56]
+    refs = [O.twod(oproblem(p), nv=(1, 1, 1), want_rho=True) for p in ps]
+    b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)} for p in ps])
+    cands = b.candidates()
+    assert len({g for g, _, _, _ in cands}) == 1
+    assert {"stage_sparse", "stage_sparse_rows"} <= {k for _, _, k, _ in cands}
+    for g, c, kern, _ in cands:
+        b.select(g, c)
+        b.run()
+        b.sync()
+        for i, (Pr, rr) in enumerate(refs):
+            assert_driven(rr)
+            assert rel(b.output(i), Pr) < TOL, (kern, i)
+            assert rel(b.rho(i), rr) < TOL, (kern, i)
+    b.close()
+
+
 def test_cli_2des_prints_reference_format(ctx, tmp_path):
     import subprocess
     txt = open("tests/golden/cpu_dimer_ohmic.key-tmpl").read()
