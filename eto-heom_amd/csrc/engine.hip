@@ -892,7 +892,8 @@ __global__ __launch_bounds__(512, 4) void stage_sparse_rows(StageArgs A) {
         // the input, the lists and the coupling gathers' tables in flight together; the
         // input into LDS (its registers free), then the neighbour rows/columns (two-phase
         // gathers: three entries and two modes per thread, any more by gather_rest).
-        // Measured on the chain grid: stage 0 14.1 -> 13.3 ms, stages 2-3 unchanged; stage 1,
+        // Measured on 256 propagations of the 10-site chain (N = 56, 1771 ADOs each): stage 0
+        // 14.1 -> 13.3 ms, stages 2-3 unchanged; stage 1,
         // whose rows hold more registers, spills with it and keeps the one-phase gathers.
         double2 xr[R];
         int ix[2];
@@ -1912,9 +1913,9 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
         std::vector<std::pair<int, std::pair<int, int>>> ranges;   // ADOs per tile -> (begin, count)
         // with the XCD order below, the instances go to the slices round-robin (member k to
         // slice k mod xcd): neighbouring grid points, which have their pulses on at the same
-        // steps, then load every XCD alike instead of a few (measured on the chain grid:
-        // RK4 steps 250-1249 in 50-step replays 3.35-3.57 s -> 3.29-3.36 s, where the steps
-        // without any pulse take 3.25 s)
+        // steps, then load every XCD alike instead of a few (measured on 256 propagations of
+        // the 10-site chain, a 64 t1 x 4 t2 grid: RK4 steps 250-1249 in 50-step replays
+        // 3.35-3.57 s -> 3.29-3.36 s, where the steps without any pulse take 3.25 s)
         std::vector<int> order(members.begin(), members.end());
         if (xcd > 1 && cls > 16) {
             order.clear();
@@ -2036,7 +2037,7 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
             return fail(EH_ESHAPE, "state tile exceeds LDS");
         for (size_t c = 0; c < g.cand.size(); c++) {
             if (use_pipe && g.cand[c].nt == -33) g.sel = (int)c;
-            if (g.cand[c].nt == -69) g.sel = (int)c;   // measured: chain grid 78.2 -> 73.9 ms per RK4 step
+            if (g.cand[c].nt == -69) g.sel = (int)c;   // measured (256 x N = 56 chain): 78.2 -> 73.9 ms per RK4 step
         }
         g.nt = g.cand[0].nt;
         g.tile_begin = g.cand[0].tile_begin;
