@@ -51,6 +51,8 @@ struct AdjArgs {
     double2* gacc;           // per instance accumulators, ncopy copies each
     const long long* gacc_off;
     int ncopy;               // tile t adds into copy t % ncopy (spreads the atomics)
+    int ntiles;              // adj_mfma64<true>: the launch's tiles (two per workgroup)
+    int half_lds;            // adj_mfma64<true>: bytes of LDS per half workgroup
     long long traj;          // trajectory mode: Yf moves by step * traj elements
     double2* slots;          // small-N tiles: one accumulator block per tile (non-returning atomic adds)
     const long long* slot_off;
@@ -584,43 +586,60 @@ __global__ __launch_bounds__(256) void adj_generic(AdjArgs J) {
     }
 }
 
-// 32 < N <= 64: one ADO per workgroup of 512 threads (8 waves, each two 16 x 16 tiles of
-// one tile row, as stage_mfma64), two workgroups per CU.  LDS holds the adjoint stage
-// input l (N x (N+1) complex), the coupling rows / columns and the bath sums; the forward
-// stage input Y is read through the caches.  On FP64 MFMA:
+// 32 < N <= 64: one ADO per 512 threads (8 waves, each two 16 x 16 tiles of one tile row, as
+// stage_mfma64).  LDS holds the adjoint stage input l (N x (N+1) complex), the coupling rows /
+// columns and the bath sums; the forward stage input Y is read through the caches.  On FP64
+// MFMA:
 //   C' = V^H l - l V^H                      (the stage update u = i C' + conj(damp) l + couplings)
 //   Gt = Y K^H - K^H Y,  K = fs l           (dL/dV, added into the instance's accumulator copy)
 // with the element-wise terms, the readout fold, q_n and the bath reductions as adj_generic.
-__global__ __launch_bounds__(512, 2) void adj_mfma64(AdjArgs J) {
+// PAIR: a 1024-thread workgroup runs two ADOs, one per half (tiles p and p + 8 of the
+// XCD-ordered list: one XCD's share, consecutive ADOs of an instance).  When both belong to one
+// instance, the second half hands its Gt to the first through its own (then unused) LDS and
+// the first adds both with one set of atomics, which cost by their number.  Every barrier is
+// reached by both halves; a half without a tile skips the work between them.
+template <bool PAIR>
+__global__ __launch_bounds__(PAIR ? 1024 : 512, PAIR ? 1 : 2) void adj_mfma64(AdjArgs J) {
     const StageArgs& A = J.S;
     extern __shared__ double2 smem[];
-    const int tile = blockIdx.x;
-    const int ii = A.tile_inst[tile];
-    const InstDev& I = A.inst[ii];
+    // the half: uniform in each wave, so a scalar (the instance data then stays in SGPRs)
+    const int half = PAIR ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 9)) : 0;
+    const int tid = (int)(threadIdx.x & 511);
+    const int tile0 = PAIR ? (int)((blockIdx.x >> 3) * 16 + (blockIdx.x & 7)) : (int)blockIdx.x;
+    const int tile = tile0 + 8 * half;
     const int step = *A.step_ctr + A.step_off;
-    if (step < 0 || step >= I.nsteps) return;
+    // both halves' instances (clamped reads for a missing second tile)
+    const int T = PAIR ? J.ntiles : (int)gridDim.x;
+    const int iiA = A.tile_inst[min(tile0, T - 1)], iiB = A.tile_inst[min(tile0 + 8, T - 1)];
+    const int ii = half ? iiB : iiA;
+    const InstDev& I = A.inst[ii];
+    const bool valid = tile < T && step >= 0 && step < I.nsteps;   // uniform in the half
+    const bool same = PAIR && tile0 + 8 < T && iiA == iiB && step >= 0 && step < I.nsteps;
+    if (!PAIR && !valid) return;
     J.Yf += (long long)step * J.traj;
     const int N = I.N, NN = N * N, d = I.d, LD = N + 1;
-    double2* sR = smem;                                               // l, N x (N+1)
-    double2* sC = smem + N * LD;                                      // 2 K N coupling rows/columns
+    double2* const sbase = reinterpret_cast<double2*>(reinterpret_cast<char*>(smem) + (size_t)half * J.half_lds);
+    double2* sR = sbase;                                              // l, N x (N+1)
+    double2* sC = sbase + N * LD;                                     // 2 K N coupling rows/columns
     double* sU = reinterpret_cast<double*>(sC + 2 * I.K * N);         // U, A, B (6 d doubles)
-    __shared__ double2 red[8];
-    const int a0 = A.tile_ado0[tile];
+    __shared__ double2 red[2][8];
+    const int a0 = A.tile_ado0[min(tile, T - 1)];
     const int st = A.stage;
     const long long base = I.state_off + (long long)a0 * NN;
     const double2* Yf = J.Yf + base;
-    for (int e = threadIdx.x; e < NN; e += 512) {
-        const int r = e / N, c = e - r * N;
-        sR[r * LD + c] = A.rin[base + e];
+    if (valid) {
+        for (int e = tid; e < NN; e += 512) {
+            const int r = e / N, c = e - r * N;
+            sR[r * LD + c] = A.rin[base + e];
+        }
+        for (int e = tid; e < 6 * d; e += 512) sU[e] = 0.0;
+        gather_couplings(A, I, a0, 1, sC, tid, 512);
     }
-    for (int e = threadIdx.x; e < 6 * d; e += 512) sU[e] = 0.0;
-    gather_couplings(A, I, a0, 1, sC, threadIdx.x, 512);
     __syncthreads();
     const double2* V = A.V + I.v_off;   // V^H of this step
-    const bool vreal = A.vreal[I.step_off + step] != 0;
+    const bool vreal = A.vreal[I.step_off + min(max(step, 0), I.nsteps - 1)] != 0;
     const double fs = st == 0 ? I.h / 6.0 : 1.0;
-    const double2 g = A.damp[I.ado_off + a0];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int ti = wave >> 1, tj0 = (wave & 1) * 2;
     const int ar = ti * 16 + li;
@@ -631,151 +650,190 @@ __global__ __launch_bounds__(512, 2) void adj_mfma64(AdjArgs J) {
         cr[t] = dbl4{0.0, 0.0, 0.0, 0.0};
         ci[t] = dbl4{0.0, 0.0, 0.0, 0.0};
     }
-    // C' = V^H l - l V^H
-    if (vreal) {
+    if (valid) {
+        const double2 g = A.damp[I.ado_off + a0];
+        // C' = V^H l - l V^H
+        if (vreal) {
 #pragma unroll 2
-        for (int k0 = 0; k0 < kmax; k0 += 4) {
-            const int kk = k0 + lk;
-            const double va = gl_at(V, ar, kk, N).x;
-            const double2 ra = lds_at(sR, ar, kk, N, LD);
+            for (int k0 = 0; k0 < kmax; k0 += 4) {
+                const int kk = k0 + lk;
+                const double va = gl_at(V, ar, kk, N).x;
+                const double2 ra = lds_at(sR, ar, kk, N, LD);
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                const int bc = (tj0 + t) * 16 + li;
-                const double vb = gl_at(V, kk, bc, N).x;
-                const double2 rb = lds_at(sR, kk, bc, N, LD);
-                cr[t] = mfma4(va, rb.x, cr[t]);
-                ci[t] = mfma4(va, rb.y, ci[t]);
-                cr[t] = mfma4(ra.x, -vb, cr[t]);
-                ci[t] = mfma4(ra.y, -vb, ci[t]);
+                for (int t = 0; t < 2; ++t) {
+                    const int bc = (tj0 + t) * 16 + li;
+                    const double vb = gl_at(V, kk, bc, N).x;
+                    const double2 rb = lds_at(sR, kk, bc, N, LD);
+                    cr[t] = mfma4(va, rb.x, cr[t]);
+                    ci[t] = mfma4(va, rb.y, ci[t]);
+                    cr[t] = mfma4(ra.x, -vb, cr[t]);
+                    ci[t] = mfma4(ra.y, -vb, ci[t]);
+                }
+            }
+        } else {
+#pragma unroll 2
+            for (int k0 = 0; k0 < kmax; k0 += 4) {
+                const int kk = k0 + lk;
+                const double2 va = gl_at(V, ar, kk, N);
+                const double2 ra = lds_at(sR, ar, kk, N, LD);
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    const int bc = (tj0 + t) * 16 + li;
+                    const double2 vb = gl_at(V, kk, bc, N);
+                    const double2 rb = lds_at(sR, kk, bc, N, LD);
+                    cr[t] = mfma4(va.x, rb.x, cr[t]);
+                    ci[t] = mfma4(va.x, rb.y, ci[t]);
+                    cr[t] = mfma4(-va.y, rb.y, cr[t]);
+                    ci[t] = mfma4(va.y, rb.x, ci[t]);
+                    cr[t] = mfma4(ra.x, -vb.x, cr[t]);
+                    ci[t] = mfma4(ra.y, -vb.x, ci[t]);
+                    cr[t] = mfma4(ra.y, vb.y, cr[t]);
+                    ci[t] = mfma4(ra.x, -vb.y, ci[t]);
+                }
             }
         }
-    } else {
-#pragma unroll 2
-        for (int k0 = 0; k0 < kmax; k0 += 4) {
-            const int kk = k0 + lk;
-            const double2 va = gl_at(V, ar, kk, N);
-            const double2 ra = lds_at(sR, ar, kk, N, LD);
-#pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                const int bc = (tj0 + t) * 16 + li;
-                const double2 vb = gl_at(V, kk, bc, N);
-                const double2 rb = lds_at(sR, kk, bc, N, LD);
-                cr[t] = mfma4(va.x, rb.x, cr[t]);
-                ci[t] = mfma4(va.x, rb.y, ci[t]);
-                cr[t] = mfma4(-va.y, rb.y, cr[t]);
-                ci[t] = mfma4(va.y, rb.x, ci[t]);
-                cr[t] = mfma4(ra.x, -vb.x, cr[t]);
-                ci[t] = mfma4(ra.y, -vb.x, ci[t]);
-                cr[t] = mfma4(ra.y, vb.y, cr[t]);
-                ci[t] = mfma4(ra.x, -vb.y, ci[t]);
-            }
-        }
-    }
-    // the stage update of the lane's 8 elements
-    double2 rg = make_double2(0.0, 0.0);
-    const bool rdo = st == 3 && a0 == 0 && readout_g(J, I, step, rg);
-    const double2* wp = st == 3 ? A.acc : ((st == 1 || st == 2) ? A.y : A.rin);
-    double2 wv[2][4];
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-        const int col = (tj0 + t) * 16 + li;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = ti * 16 + lk + 4 * r;
-            const bool in = row < N && col < N;
-            wv[t][r] = wp[(size_t)(base + (in ? row * N + col : 0))];
-        }
-    }
-    double2 qsum = make_double2(0.0, 0.0);
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-        const int col = (tj0 + t) * 16 + li;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = ti * 16 + lk + 4 * r;
-            if (row < N && col < N) {
-                const double2 x = sR[row * LD + col];
-                double ur = -ci[t][r], ui = cr[t][r];   // +i * C'
-                ur = fma(g.x, x.x, fma(-g.y, x.y, ur));
-                ui = fma(g.x, x.y, fma(g.y, x.x, ui));
-                const double2 cpl = coupling_of(I, sC, row, col);
-                const double2 yv = Yf[row * N + col];
-                const double2 qv = cconj_mul(x, yv);
-                qsum.x += fs * qv.x;
-                qsum.y += fs * qv.y;
-                if (rdo) readout_fold(J, I, ii, row, col, yv, rg, ur, ui);
-                rk_store_adj(A, I, (size_t)(base + row * N + col), x, wv[t][r], wv[t][r], ur + cpl.x, ui + cpl.y);
-            }
-        }
-    }
-    // Gt = Y K^H - K^H Y (K = fs l): P1[a][b] = sum_c Y[a][c] conj(K[b][c]),
-    // P2[a][b] = sum_c conj(K[c][a]) Y[c][b]
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-        cr[t] = dbl4{0.0, 0.0, 0.0, 0.0};
-        ci[t] = dbl4{0.0, 0.0, 0.0, 0.0};
-    }
-#pragma unroll 2
-    for (int k0 = 0; k0 < kmax; k0 += 4) {
-        const int kk = k0 + lk;
-        const double2 y1 = gl_at(Yf, ar, kk, N);        // Y[a][c]   (A operand)
-        const double2 k2 = lds_at(sR, kk, ar, N, LD);   // l[c][a]   (A operand, conjugated)
+        // the stage update of the lane's 8 elements
+        double2 rg = make_double2(0.0, 0.0);
+        const bool rdo = st == 3 && a0 == 0 && readout_g(J, I, step, rg);
+        const double2* wp = st == 3 ? A.acc : ((st == 1 || st == 2) ? A.y : A.rin);
+        double2 wv[2][4];
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
-            const int bc = (tj0 + t) * 16 + li;
-            const double2 k1 = lds_at(sR, bc, kk, N, LD);   // l[b][c]  (B operand, conjugated)
-            const double2 y2 = gl_at(Yf, kk, bc, N);        // Y[c][b]  (B operand)
-            cr[t] = mfma4(y1.x, k1.x, cr[t]);
-            ci[t] = mfma4(-y1.x, k1.y, ci[t]);
-            cr[t] = mfma4(y1.y, k1.y, cr[t]);
-            ci[t] = mfma4(y1.y, k1.x, ci[t]);
-            cr[t] = mfma4(-k2.x, y2.x, cr[t]);
-            ci[t] = mfma4(-k2.x, y2.y, ci[t]);
-            cr[t] = mfma4(-k2.y, y2.y, cr[t]);
-            ci[t] = mfma4(k2.y, y2.x, ci[t]);
+            const int col = (tj0 + t) * 16 + li;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = ti * 16 + lk + 4 * r;
+                const bool in = row < N && col < N;
+                wv[t][r] = wp[(size_t)(base + (in ? row * N + col : 0))];
+            }
+        }
+        double2 qsum = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int col = (tj0 + t) * 16 + li;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = ti * 16 + lk + 4 * r;
+                if (row < N && col < N) {
+                    const double2 x = sR[row * LD + col];
+                    double ur = -ci[t][r], ui = cr[t][r];   // +i * C'
+                    ur = fma(g.x, x.x, fma(-g.y, x.y, ur));
+                    ui = fma(g.x, x.y, fma(g.y, x.x, ui));
+                    const double2 cpl = coupling_of(I, sC, row, col);
+                    const double2 yv = Yf[row * N + col];
+                    const double2 qv = cconj_mul(x, yv);
+                    qsum.x += fs * qv.x;
+                    qsum.y += fs * qv.y;
+                    if (rdo) readout_fold(J, I, ii, row, col, yv, rg, ur, ui);
+                    rk_store_adj(A, I, (size_t)(base + row * N + col), x, wv[t][r], wv[t][r], ur + cpl.x,
+                                 ui + cpl.y);
+                }
+            }
+        }
+        // q_n = fs sum conj(l) Y over the ADO (wave sums now, n_p q_n after the barrier); the
+        // bath reductions into sU
+        for (int o = 32; o > 0; o >>= 1) {
+            qsum.x += __shfl_xor(qsum.x, o, 64);
+            qsum.y += __shfl_xor(qsum.y, o, 64);
+        }
+        if (lane == 0) red[half][wave] = qsum;
+        reduce_bath_ld(J, I, a0, sR, LD, fs, sU, tid, 512);
+        // Gt = Y K^H - K^H Y (K = fs l): P1[a][b] = sum_c Y[a][c] conj(K[b][c]),
+        // P2[a][b] = sum_c conj(K[c][a]) Y[c][b]
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            cr[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+            ci[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+        }
+#pragma unroll 2
+        for (int k0 = 0; k0 < kmax; k0 += 4) {
+            const int kk = k0 + lk;
+            const double2 y1 = gl_at(Yf, ar, kk, N);        // Y[a][c]   (A operand)
+            const double2 k2 = lds_at(sR, kk, ar, N, LD);   // l[c][a]   (A operand, conjugated)
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int bc = (tj0 + t) * 16 + li;
+                const double2 k1 = lds_at(sR, bc, kk, N, LD);   // l[b][c]  (B operand, conjugated)
+                const double2 y2 = gl_at(Yf, kk, bc, N);        // Y[c][b]  (B operand)
+                cr[t] = mfma4(y1.x, k1.x, cr[t]);
+                ci[t] = mfma4(-y1.x, k1.y, ci[t]);
+                cr[t] = mfma4(y1.y, k1.y, cr[t]);
+                ci[t] = mfma4(y1.y, k1.x, ci[t]);
+                cr[t] = mfma4(-k2.x, y2.x, cr[t]);
+                ci[t] = mfma4(-k2.x, y2.y, ci[t]);
+                cr[t] = mfma4(-k2.y, y2.y, cr[t]);
+                ci[t] = mfma4(k2.y, y2.x, ci[t]);
+            }
         }
     }
-    double2* acc = J.gacc + J.gacc_off[ii] + (long long)(tile % J.ncopy) * gacc_size(NN, d);
-    const double2* ef = J.field + 3 * (size_t)(I.step_off + step);
-    const double2 e0 = ef[0], e1 = ef[1], e2 = ef[2];
-    const bool field_on = e0.x != 0.0 || e0.y != 0.0 || e1.x != 0.0 || e1.y != 0.0 || e2.x != 0.0 || e2.y != 0.0;
+    __syncthreads();   // l, the couplings and sU are read; red and sU are written
+    if (PAIR && same) {
+        // the second half's Gt into its own LDS, the first adds it to its own
+        if (half == 1) {
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-        const int col = (tj0 + t) * 16 + li;
+            for (int t = 0; t < 2; ++t) {
+                const int col = (tj0 + t) * 16 + li;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = ti * 16 + lk + 4 * r;
-            if (row < N && col < N) {
-                const int ab = row * N + col;
-                const double2 gt = make_double2(fs * cr[t][r], fs * ci[t][r]);
-                atomic_add2(acc + ab, gt);
-                if (field_on) {
-                    atomic_add2(acc + NN + ab, cmad(make_double2(0.0, 0.0), e0, gt));
-                    atomic_add2(acc + 2 * NN + ab, cmad(make_double2(0.0, 0.0), e1, gt));
-                    atomic_add2(acc + 3 * NN + ab, cmad(make_double2(0.0, 0.0), e2, gt));
+                for (int r = 0; r < 4; ++r) {
+                    const int row = ti * 16 + lk + 4 * r;
+                    if (row < N && col < N) sR[row * LD + col] = make_double2(cr[t][r], ci[t][r]);
+                }
+            }
+        }
+        __syncthreads();
+        if (half == 0) {
+            const double2* sRB = reinterpret_cast<const double2*>(reinterpret_cast<const char*>(smem) + J.half_lds);
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int col = (tj0 + t) * 16 + li;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = ti * 16 + lk + 4 * r;
+                    if (row < N && col < N) {
+                        const double2 o = sRB[row * LD + col];
+                        cr[t][r] += o.x;
+                        ci[t][r] += o.y;
+                    }
                 }
             }
         }
     }
-    // q_n = fs sum conj(l) Y over the ADO, then n_p q_n; the bath reductions into sU
-    for (int o = 32; o > 0; o >>= 1) {
-        qsum.x += __shfl_xor(qsum.x, o, 64);
-        qsum.y += __shfl_xor(qsum.y, o, 64);
+    if (!valid) return;   // no barrier follows
+    double2* acc = J.gacc + J.gacc_off[ii] + (long long)(tile % J.ncopy) * gacc_size(NN, d);
+    if (!(PAIR && same && half == 1)) {
+        const double2* ef = J.field + 3 * (size_t)(I.step_off + step);
+        const double2 e0 = ef[0], e1 = ef[1], e2 = ef[2];
+        const bool field_on = e0.x != 0.0 || e0.y != 0.0 || e1.x != 0.0 || e1.y != 0.0 || e2.x != 0.0 || e2.y != 0.0;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int col = (tj0 + t) * 16 + li;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = ti * 16 + lk + 4 * r;
+                if (row < N && col < N) {
+                    const int ab = row * N + col;
+                    const double2 gt = make_double2(fs * cr[t][r], fs * ci[t][r]);
+                    atomic_add2(acc + ab, gt);
+                    if (field_on) {
+                        atomic_add2(acc + NN + ab, cmad(make_double2(0.0, 0.0), e0, gt));
+                        atomic_add2(acc + 2 * NN + ab, cmad(make_double2(0.0, 0.0), e1, gt));
+                        atomic_add2(acc + 3 * NN + ab, cmad(make_double2(0.0, 0.0), e2, gt));
+                    }
+                }
+            }
+        }
     }
-    if (lane == 0) red[wave] = qsum;
-    reduce_bath_ld(J, I, a0, sR, LD, fs, sU, threadIdx.x, 512);
-    __syncthreads();
     double2 qn = make_double2(0.0, 0.0);
 #pragma unroll
     for (int w = 0; w < 8; ++w) {
-        qn.x += red[w].x;
-        qn.y += red[w].y;
+        qn.x += red[half][w].x;
+        qn.y += red[half][w].y;
     }
-    for (int p = threadIdx.x; p < d; p += 512) {
+    for (int p = tid; p < d; p += 512) {
         const int np_ = J.ado[I.lk_off + (long long)a0 * d + p];
         if (np_) atomic_add2(acc + 5 * NN + p, make_double2(np_ * qn.x, np_ * qn.y));
     }
-    for (int t = threadIdx.x; t < 3 * d; t += 512) {
+    for (int t = tid; t < 3 * d; t += 512) {
         const int w = t / d, p = t - w * d;
         atomic_add2(acc + 5 * NN + (1 + w) * d + p, make_double2(sU[2 * (w * d + p)], sU[2 * (w * d + p) + 1]));
     }
@@ -914,8 +972,17 @@ static int launch_adj(const Group& g, const AdjArgs& base, hipStream_t s) {
 #undef EH_CASE
         default: {
             // every N > 16 group has one ADO per tile
-            if (adj_uses_mfma64(g))
-                hipLaunchKernelGGL(adj_mfma64, grid, dim3(512), adj_mfma64_lds(g), s, a);
+            if (adj_uses_mfma64(g)) {
+                const size_t lds = adj_mfma64_lds(g);
+                if (g.adj_pair) {   // two ADOs per 1024-thread workgroup (see adj_mfma64)
+                    a.ntiles = g.tile_count;
+                    a.half_lds = (int)((lds + 15) & ~(size_t)15);
+                    hipLaunchKernelGGL(adj_mfma64<true>, dim3((g.tile_count + 15) / 16 * 8), dim3(1024),
+                                       2 * (size_t)a.half_lds, s, a);
+                } else {
+                    hipLaunchKernelGGL(adj_mfma64<false>, grid, dim3(512), lds, s, a);
+                }
+            }
             else
                 hipLaunchKernelGGL(adj_generic, grid, block, adj_generic_lds(g), s, a);
             break;
@@ -983,7 +1050,7 @@ int batch_enable_adjoint(Batch* b, int C) {
     }
     // the N > 16 adjoint kernels stage lambda (adj_generic: also rho) and the coupling
     // rows/columns in LDS
-    for (const Group& g : im->groups) {
+    for (Group& g : im->groups) {
         if (g.nt >= 1 && g.nt <= 16) continue;
         const bool mf = adj_uses_mfma64(g);
         const size_t sh = mf ? adj_mfma64_lds(g) : adj_generic_lds(g);
@@ -992,8 +1059,15 @@ int batch_enable_adjoint(Batch* b, int C) {
                                        "workgroup (" + (mf ? "N (N+1)" : "2 N^2") + " + 2 K N complex + 48 d); " +
                                        "the device has " + std::to_string(b->ctx->lds_per_block));
         if (sh > 65536)
-            HIPCHK(hipFuncSetAttribute(mf ? (const void*)adj_mfma64 : (const void*)adj_generic,
+            HIPCHK(hipFuncSetAttribute(mf ? (const void*)adj_mfma64<false> : (const void*)adj_generic,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+        // two ADOs per workgroup where both halves' LDS fits (ETOHEOM_ADJ_NO_PAIR=1: one)
+        const size_t sh2 = 2 * ((sh + 15) & ~(size_t)15);
+        const char* np = std::getenv("ETOHEOM_ADJ_NO_PAIR");
+        g.adj_pair = mf && (long long)sh2 <= b->ctx->lds_per_block && !(np && np[0] == '1');
+        if (g.adj_pair && sh2 > 65536)
+            HIPCHK(hipFuncSetAttribute((const void*)adj_mfma64<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)sh2));
     }
     int rc;
     const size_t T = (size_t)a->total;

@@ -79,6 +79,7 @@ struct Group {
     std::vector<LaunchCfg> cand;  // forward launch candidates, cand[0] = the base configuration
     std::vector<double> cand_ms;  // autotuner timings per candidate (ms per RK4 step), empty if not tuned
     int sel = 0;                  // the candidate the forward graph launches
+    bool adj_pair = false;        // adj_mfma64 runs two ADOs per workgroup (set by the adjoint's enable)
 };
 
 // Host side of a batch (build_host_batch): the tables batch_create uploads, the
