@@ -1024,11 +1024,11 @@ __device__ __forceinline__ double2 coupling_direct(const StageArgs& A, const Ins
     return c;
 }
 
-// N in (64, 128]: one ADO per workgroup of 256 threads and nothing in LDS (rho_n alone is
-// 256 KB at N = 128).  Each wave walks its 16 x 16 output tiles; the MFMA operands (V
-// and rho_n, zero beyond N) are loaded straight from global memory through the caches,
-// and each output element's coupling term is gathered from the neighbours' same element
-// (coupling_direct).
+// N > 64: one ADO per gridDim.y workgroups of 256 threads and nothing in LDS (rho_n alone
+// is 256 KB at N = 128).  The 16 x 16 output tiles of the ADO are dealt round-robin to the
+// 4 gridDim.y waves; the MFMA operands (V and rho_n, zero beyond N) are loaded straight
+// from global memory through the caches, and each output element's coupling term is
+// gathered from the neighbours' same element (coupling_direct).
 __global__ __launch_bounds__(256) void stage_mfma_wide(StageArgs A) {
     const int tile = blockIdx.x;
     const InstDev I = A.inst[A.tile_inst[tile]];   // register copy, read before any store
@@ -1047,7 +1047,7 @@ __global__ __launch_bounds__(256) void stage_mfma_wide(StageArgs A) {
     const int li = lane & 15, lk = lane >> 4;
     const int T = (N + 15) >> 4, kmax = (N + 3) & ~3;
     const double2 z = make_double2(0.0, 0.0);
-    for (int tt = wave; tt < T * T; tt += 4) {
+    for (int tt = (int)blockIdx.y * 4 + wave; tt < T * T; tt += 4 * (int)gridDim.y) {
         const int ti = tt / T, tj = tt - ti * T;
         const int ar = ti * 16 + li, bc = tj * 16 + li;   // A-operand row, B-operand column
         dbl4 cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
@@ -1101,7 +1101,7 @@ __global__ __launch_bounds__(256) void stage_mfma_wide(StageArgs A) {
             }
         }
     }
-    if (A.fused && a0 == 0) fused_tail(A, I, step, threadIdx.x, blockDim.x);
+    if (A.fused && a0 == 0 && blockIdx.y == 0) fused_tail(A, I, step, threadIdx.x, blockDim.x);
 }
 
 // rho_0 = |0><0| for every instance, all other ADOs zero; step counter to 0
@@ -1494,7 +1494,9 @@ int launch_cfg(const Group& g, const LaunchCfg& c, const StageArgs& base, hipStr
         case -34: hipLaunchKernelGGL((stage_mfma<32, true, true>), grid, block, c.shmem, s, a); break;
         case -64: hipLaunchKernelGGL((stage_mfma<64, false>), grid, block, c.shmem, s, a); break;
         case -65: hipLaunchKernelGGL(stage_mfma64, grid, dim3(512), c.shmem, s, a); break;
-        case -128: hipLaunchKernelGGL(stage_mfma_wide, grid, block, 0, s, a); break;
+        case -128:   // ge: workgroups per ADO
+            hipLaunchKernelGGL(stage_mfma_wide, dim3(c.tile_count, std::max(c.ge, 1)), block, 0, s, a);
+            break;
         case -67: hipLaunchKernelGGL(stage_sparse, grid, dim3(512), c.shmem, s, a); break;
         case -69:   // one kernel per RK4 stage and rows per lane (7: N <= 56, 8: N <= 64)
             hipLaunchKernelGGL(sparse_rows_kernel(a.stage, sparse_rows_of(g.maxnn), c.rows1), grid, dim3(512),
@@ -1665,9 +1667,8 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
     std::vector<long long>& hado0 = hb.ado0;
     int64_t state_off = 0, p_off = 0, lk_off = 0, col_off = 0, vp_off = 0;
     int ado_off = 0, v_off = 0, r_off = 0, step_off = 0, al_off = 0;
-    // instances by N class: 1..16 (each N its own class), 17 (16 < N <= 32), 18 (32 < N <= 64)
     // instances by N class: 1..16 (each N its own class), 17 (16 < N <= 32), 18 (32 < N <= 64),
-    // 19 (64 < N <= 128)
+    // 19 (N > 64)
     std::vector<std::vector<int>> by_class(20);
     auto env_on = [](const char* v) { const char* e = std::getenv(v); return e && e[0] == '1'; };
     const bool use_mfma = !env_on("ETOHEOM_NO_MFMA");
@@ -2014,8 +2015,14 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
             add(0, 0, true, stream);
             if (use_mfma) add(-32, 0, false, stream);
             if (use_mfma && pipe_ok) add(-33, 0, false, stream);
-        } else if (cls == 19) {   // 64 < N <= 128: operands from global memory; LDS-staged VALU where it fits
-            if (use_mfma) add(-128, 0, false, stream);
+        } else if (cls == 19) {   // N > 64: operands from global memory; LDS-staged VALU where it fits
+            // the wide kernel on one workgroup per ADO, and with the ADO's 16 x 16 tiles spread
+            // over as many workgroups as give each wave at most four
+            int mn = 1;
+            while (mn * mn < maxnn) mn++;
+            const int nt16 = (mn + 15) / 16, split = (nt16 * nt16 + 15) / 16;
+            if (use_mfma) add(-128, 1, false, stream);
+            if (use_mfma && split > 1) add(-128, split, false, stream);
             add(0, 0, false, stream);
         } else {   // 32 < N <= 64: stage_mfma64 gathers columns directly (measured faster at N = 56)
             if (use_mfma) add(-65, 0, false, stream);
@@ -2038,6 +2045,10 @@ int build_host_batch(eh_ctx* ctx, int n, const eh_instance* in, Batch* b, HostBa
         for (size_t c = 0; c < g.cand.size(); c++) {
             if (use_pipe && g.cand[c].nt == -33) g.sel = (int)c;
             if (g.cand[c].nt == -69) g.sel = (int)c;   // measured (256 x N = 56 chain): 78.2 -> 73.9 ms per RK4 step
+            // the wide kernel over several workgroups per ADO where the ADOs alone leave the chip
+            // short of work (measured per RK4 step: N = 137 x 561 ADOs 3.66 -> 2.80 ms, N = 256 x
+            // 7 ADOs 6.84 -> 0.61 ms; N = 67 x 2300 ADOs 1.90 one per ADO, 1.92 two)
+            if (g.cand[c].nt == -128 && g.cand[c].ge > 1 && g.cand[c].tile_count < 8 * ctx->n_cu) g.sel = (int)c;
         }
         g.nt = g.cand[0].nt;
         g.tile_begin = g.cand[0].tile_begin;
@@ -2373,7 +2384,8 @@ std::string batch_tuning_report(const Batch* b) {
             char line[256];
             std::snprintf(line, sizeof(line),
                           "group %zu cand %zu maxN^2=%d instances=%d %s tile=%d tiles=%d cols=%d stream=%d late=%d ms_per_step=%.6f%s\n",
-                          gi, c, g.maxnn, g.inst_count, kn, x.nt > 0 ? x.ge : 0, x.tile_count, x.cols ? 1 : 0,
+                          gi, c, g.maxnn, g.inst_count, kn, (x.nt > 0 || x.nt == -128) ? x.ge : 0, x.tile_count,
+                          x.cols ? 1 : 0,
                           x.stream ? 1 : 0, x.late ? 1 : 0,
                           g.cand_ms.empty() ? -1.0 : g.cand_ms[c], (int)c == g.sel ? " *" : "");
             out += line;

@@ -56,7 +56,8 @@ struct LaunchCfg {
                               // -33: stage_mfma_pipe; -128: stage_mfma_wide; -67: stage_sparse;
                               // -69: stage_sparse_rows (stage 1: stage_sparse unless rows1);
                               // 0: stage_generic
-    int ge;                   // stage_small: state elements per tile (256, 512 or 1024)
+    int ge;                   // stage_small: state elements per tile (256, 512 or 1024);
+                              // stage_mfma_wide: workgroups per ADO
     bool cols;                // keep coupled-column copies (contiguous column gathers)
     bool stream;              // non-temporal state stores (StageArgs::stream)
     bool late = false;        // stage_small<N, 512, true>: state operand loaded after the barrier

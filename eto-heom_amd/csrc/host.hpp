@@ -90,8 +90,9 @@ int build_schedule(const eh_system& sys, int mode, const int nv[3], Schedule& s)
 void rng_stream(unsigned long seed, int count, uint32_t* out);
 int draw_samples(const Problem& k, int n_sample, std::vector<cplx>& H, std::vector<cplx>& X);
 
-// Largest system dimension the stage kernels take.
-constexpr int kMaxN = 128;
+// Largest system dimension the stage kernels take (the key parser's bound; an instance's
+// state must also stay below 2^31 elements).
+constexpr int kMaxN = 2048;
 
 // One instance as checked on the host (plan.cpp): sizes and step counts.
 struct InstPlan {

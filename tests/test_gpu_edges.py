@@ -1,9 +1,11 @@
 """GPU parity at the edges of the supported shapes, against the CPU oracle (<= 1e-10
 normwise relative, FP64):
 
-* N = 64, the largest system (no padding in the 64 x 64 MFMA tiles), with a random
-  Hermitian H and random dipoles; N = 60 with a banded H and sparse dipoles on every
-  launch candidate (the sparse kernels at eight rows per lane);
+* N = 64 (no padding in the 64 x 64 MFMA tiles), with a random Hermitian H and random
+  dipoles; N = 60 with a banded H and sparse dipoles on every launch candidate (the
+  sparse kernels at eight rows per lane);
+* N > 64 on the wide kernel, one or several workgroups per ADO: the 11-site aggregate
+  (N = 67), random systems of N = 128 and 256, the 16-site aggregate (N = 137);
 * L = 0: the hierarchy is rho_0 alone (no neighbour links);
 * K = 0: no bath at all (d = 0, one ADO), including N = 1;
 * instances of different lengths and N classes in one batch, each read out to its own
@@ -211,11 +213,11 @@ def test_n67_eleven_site_aggregate_every_candidate(ctx):
     b.close()
 
 
-def test_n128_random_hermitian_system(ctx):
-    """N = 128, the largest system: random Hermitian H, three baths, L = 1, 20 steps."""
-    base = configs.chain(L=1, nsteps=20)
-    rng = np.random.default_rng(128)
-    N = 128
+def random_system(N, seed, nsteps, taus=(2.0, 5.0, 8.0)):
+    """Random Hermitian H (ground state uncoupled), random ground-to-excited dipoles, three
+    baths of two modes on sites 1..3, L = 1 (7 ADOs), strong pulses."""
+    base = configs.chain(L=1, nsteps=nsteps)
+    rng = np.random.default_rng(seed)
     A = rng.normal(size=(N, N)) * 0.02
     H = np.diag(rng.uniform(2.2, 2.45, N)) + (A + A.T) / 2
     H[0, :] = 0.0
@@ -226,7 +228,50 @@ def test_n128_random_hermitian_system(ctx):
         X[q, 0, 1:] = v
         X[q, 1:, 0] = v
     p = strong(like(base, H=H.astype(complex), X=X, alpha=base.alpha[:3], alpha_t=base.alpha_t[:3],
-                    gamma=base.gamma[:3]))
-    assert p.N == 128
+                    gamma=base.gamma[:3]), taus=taus)
+    assert p.N == N and p.num_ado() == 7
+    return p
+
+
+def every_candidate(ctx, p, splits):
+    """p on every launch candidate against the oracle; the wide kernel must be offered with
+    exactly these workgroups per ADO."""
+    b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)}])
+    cands = b.candidates()
+    assert sorted(t for _, _, k, t in cands if k == "stage_mfma_wide") == sorted(splits)
+    Pr, rr = O.twod(oproblem(p), nv=(1, 1, 1), want_rho=True)
+    assert np.max(np.abs(rr[1:])) > 1e-12 * np.max(np.abs(rr[0]))
+    for g, c, kern, t in cands:
+        print(f"N={p.N} candidate {c}: {kern} {t}", flush=True)
+        b.select(g, c)
+        b.run()
+        b.sync()
+        assert len(b.output(0)) == len(Pr) > 0
+        assert rel(b.output(0), Pr) < TOL, (kern, t)
+        assert rel(b.rho(0), rr) < TOL, (kern, t)
+    b.close()
+
+
+def test_n128_random_hermitian_system(ctx):
+    """N = 128: random Hermitian H, three baths, L = 1, 20 steps, on the wide kernel with
+    one workgroup per ADO and with its 64 tiles over four; and through the single-propagation
+    entry point."""
+    p = random_system(128, 128, 20)
+    every_candidate(ctx, p, [1, 4])
     check(ctx, p)
+
+
+def test_n137_sixteen_site_aggregate_every_candidate(ctx):
+    """N = 137 (16 sites: ground + 16 singles + 120 doubles), L = 1 (33 ADOs), 8 steps
+    with the three strong pulses inside them: the wide kernel with one workgroup per ADO
+    and with each ADO's 81 tiles over six."""
+    p = strong(configs.aggregate(16, 1, nsteps=8), taus=(0.5, 1.5, 2.5))
+    assert p.N == 137 and p.num_ado() == 33
+    every_candidate(ctx, p, [1, 6])
+
+
+def test_n256_random_hermitian_system(ctx):
+    """N = 256 (16 x 16 tiles of 16 x 16 per ADO), 8 steps with the pulses inside them: the
+    wide kernel with one workgroup per ADO and with 16 per ADO."""
+    every_candidate(ctx, random_system(256, 256, 8, taus=(0.5, 1.5, 2.5)), [1, 16])
 

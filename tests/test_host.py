@@ -277,18 +277,23 @@ def test_batch_plan_bytes():
 
 
 def test_batch_plan_system_size_range():
-    """N up to 128 is accepted (the N > 64 kernels keep nothing of rho in LDS); 129 is
-    refused on the host with EH_ESHAPE."""
-    def item(N):
+    """N up to 2048 (the key parser's bound) is accepted: the N > 64 kernels keep nothing
+    of rho in LDS.  2049 is refused on the host with EH_ESHAPE, and an instance of 2^31
+    state elements or more (32-bit in-instance offsets) with EH_ERANGE."""
+    def item(N, L=1):
         al = np.full((3, 2), 3.3e-5 - 1.5e-5j)
         p = E.Problem(H=np.eye(N, dtype=complex), alpha=al, alpha_t=al.conj(), gamma=np.full((3, 2), 0.023 + 0j),
-                      L=1, X=np.zeros((4, N, N), complex), site_offset=1,
+                      L=L, X=np.zeros((4, N, N), complex), site_offset=1,
                       pulses=np.array([[1e-4 / HB, 40.0, 15.0, 2.27]] * 3), t_start=0.0, t_end=5.0,
                       step_size=0.5, print_step=0.5)
         return {"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)}
-    for N in (65, 67, 128):
+    for N in (65, 67, 128, 137, 256, 2048):
         pl = E.batch_plan([item(N)])
         assert pl["total_elems"] == 7 * N * N
     with pytest.raises(E.EtoHeomError) as ei:
-        E.batch_plan([item(129)])
-    assert ei.value.code == -3 and "1..128" in str(ei.value)
+        E.batch_plan([item(2049)])
+    assert ei.value.code == -3 and "1..2048" in str(ei.value)
+    # d = 6, L = 7: 1716 ADOs of 2048^2 elements
+    with pytest.raises(E.EtoHeomError) as ei:
+        E.batch_plan([item(2048, L=7)])
+    assert "2^31" in str(ei.value)
