@@ -5,19 +5,20 @@ The default workload is the largest single-GPU configuration: the 10-site chain
 (1771 ADOs), as a batch of 64 t1 x 4 t2 = 256 propagations of 3000 RK4 steps of
 0.5 fs (about 91 GB of hierarchy state), each with its three Gaussian pulses.
 
-A bench "step" is one twentieth of the workload's propagation: the next
-S = ceil(max_steps / 20) RK4 steps of every instance (one replay of the forward
-graph captured with S steps; V(t) built and P(t) read out on the device), with
-the state resident in HBM.  The timed region starts from a reset, so 20 timed
-steps are exactly one whole propagation of the batch, pulse windows and every
-readout included; more steps wrap round (reset, step 0 again).  `value` is
-ADO-updates/s (one ADO advanced one full RK4 step).  Other workloads:
---workload heptamer | dimer | grid2d | chain10 | trimer | trimer_grad.
+A bench "step" is the next S RK4 steps of every instance (one replay of the
+forward graph captured with S steps; V(t) built and P(t) read out on the
+device), with the state resident in HBM: S = 50 for the chain grid, whose 5
+warm-up steps run RK4 steps 0-249 from a reset and whose 20 timed steps run
+250-1249 (the first pulse windows among them); S = ceil(max_steps / 20) for the
+other workloads.  --full times one whole propagation from a reset instead.
+`value` is ADO-updates/s (one ADO advanced one full RK4 step).  Other workloads:
+--workload heptamer | dimer | grid2d | chain10 | trimer | trimer_grad | chain_grad.
 
 Printed as one JSON line with the roofline of the dominant kernel (the fused
-RHS+RK4 stage kernel, timed per launch with HIP events on its stream) and the
-CPU oracle timed on this host over a bounded sample of the same workload, one
-process per core.
+RHS+RK4 stage kernel, timed per launch with HIP events on its stream), the CPU
+oracle timed on this host over a bounded sample of the same workload, one
+process per core, and for the chain grid the adjoint leg: the trimer grid
+gradient (48 propagations, L = 5) against its forward and central differences.
 """
 import argparse
 import json
@@ -308,6 +309,8 @@ def main():
                     help="only the per-launch timing pass (--profile-steps eager steps from a reset); for "
                          "rocprofv3 runs whose kernel averages must match the roofline figure")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint-leg", action="store_true",
+                    help="skip the trimer grid gradient the default workload reports beside its value")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-seed", type=int, default=0, help=argparse.SUPPRESS)
@@ -454,6 +457,9 @@ def main():
                    "complex_v_share_timed": complex_v_share(items, first, min(last, max_steps)),
                    "complex_v_share_whole": complex_v_share(items, 0, max_steps),
                    "propagations_per_s": value / (full_ups / len(items)),
+                   **({"grid_points_per_s": value / (full_ups / len(items)),
+                       "grid_points_note": "one propagation per (t1, t2) point of the grid"}
+                      if a.workload.startswith("chain_grid") else {}),
                    "gpu_event_ms_timed": float(sum(gpu_ms)), "device": ctx.name, "finite": finite,
                    "readouts_complete": done, "groups": batch.tuning_report().splitlines(),
                    "setup_s": t_setup,
@@ -484,10 +490,34 @@ def main():
         per_prop["batched_over_per_propagation"] = rec["config"]["propagations_per_s"] / per_prop["propagations_per_s"]
         rec["config"]["per_propagation_path"] = per_prop
     batch.close()
+    if a.workload == "chain_grid" and not a.no_adjoint_leg:
+        rec["adjoint_leg"] = adjoint_leg()
     if not a.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(a.workload, a.cpu_seconds)
     print(json.dumps(rec))
     return 0
+
+
+def adjoint_leg(timeout=600):
+    """The metric's adjoint leg: the trimer grid gradient (16 x 16 (t1, t3) rephasing grid,
+    48 propagations, L = 5, checkpoints every 32 steps) timed by this script's trimer_grad
+    workload in a child process (2 warm-up and 3 timed gradients); its progress goes to
+    stderr as it runs."""
+    cmd = [sys.executable, os.path.abspath(__file__), "--workload", "trimer_grad", "--warmup", "2", "--steps", "3",
+           "--no-cpu"]
+    print("bench: adjoint leg (trimer grid gradient) in a child process", file=sys.stderr, flush=True)
+    try:
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=None, text=True, timeout=timeout)
+    except subprocess.TimeoutExpired:
+        return {"error": f"trimer_grad did not finish in {timeout} s"}
+    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"error": f"trimer_grad exited with {r.returncode}"}
+    d = json.loads(lines[-1])
+    leg = {"workload": d["config"]["workload"], "ado_updates_per_s": d["value"]}
+    leg.update(d["config"]["adjoint"])
+    leg["grid_points_per_s"] = d["config"].get("grid_points_per_s")
+    return leg
 
 
 if __name__ == "__main__":
