@@ -12,6 +12,11 @@ and the differentiable grid loss used for parameter gradients.
   files gen_2d_spectrum.py reads (gen-2d-spectrum/gen_2d_spectrum.py:26-61).
 * `grid_loss` is the (t1, t3) rephasing-grid L2 loss whose gradient w.r.t.
   site energies, couplings, dipoles and ETO parameters comes from the adjoint.
+* `spectrum_2d` forms the 2D spectrum from the signals as
+  gen-2d-spectrum/gen_2d_spectrum.py:26-66 does from the out files (before its
+  smoothing, interpolation and plot), with torch.fft on the signals' device, and
+  `spectral_loss` is the L2 loss on it; both are differentiable, so a spectral
+  loss's gradient also comes from the adjoint.
 """
 from __future__ import annotations
 
@@ -161,6 +166,33 @@ def grid_signal(params, spec, propagate=None):
 
 def grid_loss(S, target):
     return torch.sum(torch.abs(S - target) ** 2) / torch.sum(torch.abs(target) ** 2)
+
+
+def spectrum_2d(rephasing, non_rephasing=None, size=(1000, 61)):
+    """The 2D spectrum as gen-2d-spectrum/gen_2d_spectrum.py:26-66 forms it.
+
+    Each argument holds the signals S[i, j] of one set of delays (i: the delay, in the
+    ascending order the script reads the out files; j: the readout).  The script reads
+    the signal of delay i into column i of a size[0] x size[1] grid, multiplied by i
+    (D[j, i] = i S[i, j], zero past the signal's readouts and the last delay, readouts
+    beyond size[0] dropped), and sums Re fftshift(fft2(D)) of the rephasing and the
+    non-rephasing grid.  Complex128 tensors in, a float64 tensor out, on the signals'
+    device (torch.fft: hipFFT on the GPU); differentiable."""
+    def one(S):
+        if S.dim() != 2 or S.shape[0] > size[1]:
+            raise ValueError(f"spectrum_2d: signals {tuple(S.shape)} do not fit a {size[0]} x {size[1]} grid")
+        m = min(S.shape[1], size[0])
+        D = torch.zeros(size, dtype=torch.complex128, device=S.device)
+        D[:m, : S.shape[0]] = 1j * S[:, :m].transpose(0, 1).to(torch.complex128)
+        return torch.fft.fftshift(torch.fft.fft2(D)).real
+
+    A = one(rephasing)
+    return A if non_rephasing is None else A + one(non_rephasing.to(A.device))
+
+
+def spectral_loss(A, target):
+    """Relative L2 distance of two 2D spectra (real tensors)."""
+    return torch.sum((A - target) ** 2) / torch.sum(target ** 2)
 
 
 # ---------------------------------------------------------------- the 10-site chain grid

@@ -1,7 +1,8 @@
 """The torch.autograd path: gradient of a 2DES (t1, t3) grid loss w.r.t. physical
 parameters (site energies, couplings, dipoles, ETO a/b/g/w, reorganisation scale),
 adjoint on the GPU, checked against Richardson central FD of the same loss built
-on the CPU oracle (relative <= 1e-6)."""
+on the CPU oracle (relative <= 1e-6); the same for a loss on the 2D spectrum formed
+by torch.fft on the GPU."""
 import numpy as np
 import pytest
 import torch
@@ -58,6 +59,47 @@ def test_grid_gradient_matches_oracle_fd():
     checks = [("eps", 0, 1e-4), ("J", 1, 1e-3), ("mu", (0, 2), 1e-3), ("g", 0, 1e-3), ("w", 0, 0.3),
               ("lam", 0, 1e-3), ("a", 0, 0.1)]
     for name, idx, rel in checks:
+        base = params[name].detach().clone()
+        h0 = rel * max(abs(float(base[idx])), 0.5 * float(base.abs().max()))
+
+        def central(h):
+            a, b = base.clone(), base.clone()
+            a[idx] += h
+            b[idx] -= h
+            return (L_cpu({name: a}) - L_cpu({name: b})) / (2 * h)
+
+        ref = (4 * central(h0 / 2) - central(h0)) / 3
+        got = float(params[name].grad[idx])
+        assert abs(got - ref) <= 1e-6 * abs(ref), (name, idx, got, ref)
+
+
+def test_spectral_loss_gradient_matches_oracle_fd():
+    """A loss on the 2D spectrum (spectrum_2d of the rephasing grid, zero-padded to 8 x 4,
+    formed by torch.fft on the GPU) differentiated through the FFT into the adjoint equals
+    Richardson central FD of the same loss on the CPU oracle's signals (relative <= 1e-6)."""
+    ctx = E.Context(0)
+    spec = SP.grid_spec(ctx, n_t1=2, t1_step=10.0, t2=20.0, n_t3=5, t3_step=5.0, L=2, checkpoint=16, taup=8.0,
+                        E0_cm=200.0)
+    dev = torch.device("cuda", 0)
+    size = (8, 4)
+    with torch.no_grad():
+        target = SP.spectrum_2d(SP.grid_signal(SP.trimer_params(perturb=5), spec, propagate=oracle_propagate),
+                                size=size)
+    params = SP.trimer_params()
+    A = SP.spectrum_2d(SP.grid_signal(params, spec).to(dev), size=size)
+    assert A.device == dev
+    loss = SP.spectral_loss(A, target.to(dev))
+    loss.backward()
+
+    def L_cpu(over):
+        p = {k: v.detach().clone() for k, v in params.items()}
+        p.update(over)
+        with torch.no_grad():
+            S = SP.grid_signal(p, spec, propagate=oracle_propagate)
+            return float(SP.spectral_loss(SP.spectrum_2d(S, size=size), target))
+
+    assert float(loss.detach()) == pytest.approx(L_cpu({}), rel=1e-8)
+    for name, idx, rel in [("eps", 0, 1e-4), ("J", 1, 1e-3), ("mu", (0, 2), 1e-3), ("g", 0, 1e-3)]:
         base = params[name].detach().clone()
         h0 = rel * max(abs(float(base[idx])), 0.5 * float(base.abs().max()))
 
