@@ -458,7 +458,9 @@ def main():
                    "complex_v_share_whole": complex_v_share(items, 0, max_steps),
                    "propagations_per_s": value / (full_ups / len(items)),
                    **({"grid_points_per_s": value / (full_ups / len(items)),
-                       "grid_points_note": "one propagation per (t1, t2) point of the grid"}
+                       "grid_points_note": "the workload's grid has one propagation (the p111 ordering) per "
+                                           "(t1, t2) point; the adjoint leg's trimer grid runs p111/p110/p101 "
+                                           "triplets and counts a point per triplet"}
                       if a.workload.startswith("chain_grid") else {}),
                    "gpu_event_ms_timed": float(sum(gpu_ms)), "device": ctx.name, "finite": finite,
                    "readouts_complete": done, "groups": batch.tuning_report().splitlines(),
