@@ -633,7 +633,7 @@ __global__ __launch_bounds__(PAIR ? 1024 : 512, PAIR ? 1 : 2) void adj_mfma64(Ad
             sR[r * LD + c] = A.rin[base + e];
         }
         for (int e = tid; e < 6 * d; e += 512) sU[e] = 0.0;
-        gather_couplings(A, I, a0, 1, sC, tid, 512);
+        gather_couplings<true>(A, I, a0, 1, sC, tid, 512);
     }
     __syncthreads();
     const double2* V = A.V + I.v_off;   // V^H of this step

@@ -182,14 +182,18 @@ __device__ __forceinline__ void rk_store(const StageArgs& A, const InstDev& I, s
 
 // address of element (a, b) = (pos, s) [rc = 1] or (s, pos) [rc = 0] of neighbour mu
 // for bath j: columns from the column copy when the stage has one
-template <int NT = 0>
+// U24: mu and the ADO's size N^2 (and K N) below 2^24, so the full-rate 24-bit multiply is
+// exact; true for N > 16, where an instance's M N^2 < 2^31 elements bounds M below 2^23.
+template <int NT = 0, bool U24 = false>
 __device__ __forceinline__ const double2* nb_elem(const StageArgs& A, const InstDev& I, int mu, int j, int rc,
                                                   int pos) {
     // 32-bit offsets inside the instance (an instance holds < 2^32 elements), one 64-bit add
     const int N = NT > 0 ? NT : I.N, s = j + I.so;
     // the small-N kernels (NT > 0) keep no column copies (measured: no gain below N = 16)
-    if (NT == 0 && rc == 1 && A.cin) return (A.cin + I.col_off) + (unsigned)((mu * I.K + j) * N + pos);
-    return (A.rin + I.state_off) + (unsigned)(mu * N * N + (rc == 0 ? s * N + pos : pos * N + s));
+    if (NT == 0 && rc == 1 && A.cin)
+        return (A.cin + I.col_off) + (unsigned)((U24 ? __umul24(mu, I.K * N) : mu * I.K * N) + j * N + pos);
+    return (A.rin + I.state_off) +
+           (unsigned)((U24 ? __umul24(mu, N * N) : mu * N * N) + (rc == 0 ? s * N + pos : pos * N + s));
 }
 
 // the one state operand of rk_store besides the input for element gi (Y in stages 1-2,
@@ -206,22 +210,39 @@ __device__ __forceinline__ double2 rk_operand(const StageArgs& A, size_t gi, dou
 //   cp[(j*2 + 1)*N + a] = sum_k -i x rho_{n+e_p}[a][s] - i alpha~ y rho_{n-e_p}[a][s]     (column s)
 // with p = j + k K.  Every load is independent, so a thread issues all of its
 // 2 Km gathers before it waits.
+// BIG: a kernel of one ADO per tile (na = 1) and N > 16, which splits e by a float
+// reciprocal (e < 2 K N <= 2^13: exact, as entry_split states) and addresses the neighbours
+// with 24-bit products (nb_elem's U24)
+template <bool BIG = false>
 __device__ __forceinline__ void gather_couplings(const StageArgs& A, const InstDev& I, int n0, int na, double2* cp,
                                                  int tid, int nthr) {
     const int N = I.N, K = I.K, Km = I.Km, d = I.d;
     const int per = 2 * K * N;
-    const int total = per * na;
+    const int total = BIG ? per : per * na;
+    const float inv2N = 1.0f / (float)(2 * N);
     for (int e = tid; e < total; e += nthr) {
-        const int g = e / per, eg = e - g * per;
-        const int j = eg / (2 * N), rem = eg - j * 2 * N;
-        const int rc = rem / N, pos = rem - rc * N;
+        int g, j, rc, pos;
+        if constexpr (BIG) {
+            g = 0;
+            j = (int)(((float)e + 0.5f) * inv2N);
+            const int rem = e - j * 2 * N;
+            rc = rem >= N ? 1 : 0;
+            pos = rem - rc * N;
+        } else {
+            g = e / per;
+            const int eg = e - g * per;
+            j = eg / (2 * N);
+            const int rem = eg - j * 2 * N;
+            rc = rem / N;
+            pos = rem - rc * N;
+        }
         const long long row0 = I.lk_off + (long long)(n0 + g) * d;
         double2 acc = make_double2(0.0, 0.0);
         for (int k = 0; k < Km; ++k) {
             const long long q = row0 + j + k * K;
             const int mu = A.nb_up[q], md = A.nb_dn[q];
-            const double2 xu = mu >= 0 ? *nb_elem(A, I, mu, j, rc, pos) : make_double2(0.0, 0.0);
-            const double2 xd = md >= 0 ? *nb_elem(A, I, md, j, rc, pos) : make_double2(0.0, 0.0);
+            const double2 xu = mu >= 0 ? *nb_elem<0, BIG>(A, I, mu, j, rc, pos) : make_double2(0.0, 0.0);
+            const double2 xd = md >= 0 ? *nb_elem<0, BIG>(A, I, md, j, rc, pos) : make_double2(0.0, 0.0);
             const double x = A.cx[q], y = A.cy[q];
             const double2 al = rc == 0 ? A.alpha[I.al_off + j * Km + k] : A.alpha_t[I.al_off + j * Km + k];
             // up:   row  i x,  column -i x;   down: row  i alpha y,  column -i alpha~ y
@@ -250,13 +271,15 @@ struct NbIdx {
 };
 
 // entry e of a tile's coupling rows/columns, e = ((g K + j) 2 + rc) N + pos, split into
-// (ADO g, bath j, row/column rc, position pos).  NT > 0: N is a compile-time constant;
-// the division by the runtime K goes through a float reciprocal (exact here: the
-// quotient is at least 0.5/K from an integer and g K + j < 2^16).
+// (ADO g, bath j, row/column rc, position pos).  NT > 0: N is a compile-time constant.
+// The divisions by a runtime N and K go through float reciprocals: (x + 0.5) / n lies at
+// least 0.5 / n from an integer, and for x < 2^21 the float product's error is below that
+// (the N > 16 callers have one ADO per tile: e < 2 K N <= 2^13; g K + j < 2^16).
 template <int NT>
-__device__ __forceinline__ void entry_split(int e, int N, int K, float invK, int& g, int& j, int& rc, int& pos) {
+__device__ __forceinline__ void entry_split(int e, int N, int K, float invN, float invK, int& g, int& j, int& rc,
+                                            int& pos) {
     const int n = NT > 0 ? NT : N;
-    const int q = e / n;
+    const int q = NT > 0 ? e / NT : (int)(((float)e + 0.5f) * invN);
     pos = e - q * n;
     rc = q & 1;
     const int gj = q >> 1;
@@ -270,13 +293,13 @@ __device__ __forceinline__ void gather_idx(const StageArgs& A, const InstDev& I,
     // raw values only: masking them here would make the thread wait for each load
     const int N = NT > 0 ? NT : I.N, K = I.K, Km = I.Km, d = I.d;
     const int per = 2 * K * N, total = per * na;
-    const float invK = 1.0f / (float)K;
+    const float invN = 1.0f / (float)N, invK = 1.0f / (float)K;
 #pragma unroll
     for (int r = 0; r < EC; ++r) {
         const int e = tid + r * nthr;
         const int ec = e < total ? e : total - 1;
         int g, j, rc, pos;
-        entry_split<NT>(ec, N, K, invK, g, j, rc, pos);
+        entry_split<NT>(ec, N, K, invN, invK, g, j, rc, pos);
         const unsigned q0 = (unsigned)((n0 + g) * d + j);   // row in the instance's tables
 #pragma unroll
         for (int k = 0; k < KMX; ++k) {
@@ -296,7 +319,7 @@ __device__ __forceinline__ void gather_data(const StageArgs& A, const InstDev& I
                                             const NbIdx<EC, KMX>& X, double2* cp) {
     const int N = NT > 0 ? NT : I.N, K = I.K, Km = I.Km;
     const int per = 2 * K * N, total = per * na;
-    const float invK = 1.0f / (float)K;
+    const float invN = 1.0f / (float)N, invK = 1.0f / (float)K;
     double2 xu[EC][KMX], xd[EC][KMX];
     int rcs[EC];
 #pragma unroll
@@ -304,13 +327,14 @@ __device__ __forceinline__ void gather_data(const StageArgs& A, const InstDev& I
         const int e = tid + r * nthr;
         const int ec = e < total ? e : total - 1;
         int g, j, rc, pos;
-        entry_split<NT>(ec, N, K, invK, g, j, rc, pos);
+        entry_split<NT>(ec, N, K, invN, invK, g, j, rc, pos);
         rcs[r] = rc;
 #pragma unroll
         for (int k = 0; k < KMX; ++k) {
+            // NT == 0: the N > 16 kernels (24-bit neighbour offsets, as nb_elem states)
             const int mu = X.up[r][k], md = X.dn[r][k];
-            xu[r][k] = *nb_elem<NT>(A, I, mu > 0 ? mu : 0, j, rc, pos);
-            xd[r][k] = *nb_elem<NT>(A, I, md > 0 ? md : 0, j, rc, pos);
+            xu[r][k] = *nb_elem<NT, NT == 0>(A, I, mu > 0 ? mu : 0, j, rc, pos);
+            xd[r][k] = *nb_elem<NT, NT == 0>(A, I, md > 0 ? md : 0, j, rc, pos);
         }
     }
 #pragma unroll

@@ -312,7 +312,7 @@ __global__ __launch_bounds__(256, NP == 32 ? 4 : 2) void stage_mfma(StageArgs A)
                 }
             }
         }
-        gather_couplings(A, I, a0, 1, sC, threadIdx.x, 256);
+        gather_couplings<true>(A, I, a0, 1, sC, threadIdx.x, 256);
     }
 #pragma unroll
     for (int q = 0; q < SPT; ++q) {
@@ -435,7 +435,7 @@ __device__ __forceinline__ void mfma64_tile(const StageArgs& A, int tile, int st
         const double2 t = A.rin[base + (in ? r * N + c : 0)];
         xr[q] = in ? t : make_double2(0.0, 0.0);
     }
-    gather_couplings(A, I, a0, 1, sC, threadIdx.x, 512);
+    gather_couplings<true>(A, I, a0, 1, sC, threadIdx.x, 512);
 #pragma unroll
     for (int q = 0; q < SPT; ++q) {
         const int e = threadIdx.x + q * 512;
@@ -593,7 +593,7 @@ __global__ __launch_bounds__(512, 4) void stage_sparse(StageArgs A) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) ix[h] = A.sp_idx[pbase + min(tid + h * 512, N * CAP - 1)];
     const int cnt = A.sp_cnt[I.sp_off + min(tid, N - 1)];
-    gather_couplings(A, I, a0, 1, sC, tid, 512);
+    gather_couplings<true>(A, I, a0, 1, sC, tid, 512);
     double vc[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -783,7 +783,9 @@ __device__ __forceinline__ void sx_rows(const StageArgs& A, const InstDev& I, in
                                         bool sym, double2 (&wv)[R], double2 (&p1)[R], double2 (&tq)[R]) {
     constexpr int CAP = kSpCap, PS = kSpCap + 1;
     const int N = I.N, LD = N + 1;
-    const int tid = threadIdx.x, b = tid & 63, w = tid >> 6;
+    // the wave index (and so every row a = w + 8 i) as a scalar: row offsets go to the SALU
+    // (stage 1 measured slower with it)
+    const int tid = threadIdx.x, b = tid & 63, w = S == 1 ? tid >> 6 : __builtin_amdgcn_readfirstlane(tid >> 6);
     const int bc = b < N ? b : 0, l16 = b & (CAP - 1);
     const double2* V = A.V + I.v_off;
     // the state operand of the lane's elements (Y in stage 2, ACC in stage 3; stage 1 loads Y
@@ -823,8 +825,8 @@ __device__ __forceinline__ void sx_rows(const StageArgs& A, const InstDev& I, in
         } else {
             // V[a][c] and V[c][a] of the row's entries into this wave's slots (the Re V list
             // is not read with complex V), then read back as LDS broadcasts
-            const int cs = cr >= 0 ? cr : 0;
-            const double2 u = V[(long long)a * N + cs], d = V[(long long)cs * N + a];
+            const int cs = cr >= 0 ? cr : 0;   // a, cs < N: 24-bit products
+            const double2 u = V[__umul24(a, N) + cs], d = V[__umul24(cs, N) + a];
             double2* slot = reinterpret_cast<double2*>(L.sVr) + w * 2 * CAP;
             if (b < CAP) {
                 slot[b] = u;
@@ -881,7 +883,8 @@ __global__ __launch_bounds__(512, 4) void stage_sparse_rows(StageArgs A) {
     extern __shared__ double2 smem[];
     const int step = *A.step_ctr + A.step_off;
     at_step(A, step);
-    const int tile = blockIdx.x, tid = threadIdx.x, b = tid & 63, w = tid >> 6;
+    const int tile = blockIdx.x, tid = threadIdx.x, b = tid & 63;
+    const int w = S == 1 ? tid >> 6 : __builtin_amdgcn_readfirstlane(tid >> 6);   // as in sx_rows
     const InstDev I = ro(A.inst, ro(A.tile_inst, tile));   // scalar loads, then registers
     if (step >= I.nsteps) return;
     const int N = I.N, LD = N + 1;
