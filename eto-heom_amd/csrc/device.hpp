@@ -280,11 +280,11 @@ __device__ __forceinline__ void entry_split(int e, int N, int K, float invN, flo
                                             int& pos) {
     const int n = NT > 0 ? NT : N;
     const int q = NT > 0 ? e / NT : (int)(((float)e + 0.5f) * invN);
-    pos = e - q * n;
+    pos = e - __mul24(q, n);   // small operands: full-rate 24-bit products
     rc = q & 1;
     const int gj = q >> 1;
     g = (int)(((float)gj + 0.5f) * invK);
-    j = gj - g * K;
+    j = gj - __mul24(g, K);
 }
 
 template <int EC, int KMX, int NT = 0>
@@ -393,13 +393,14 @@ __device__ __forceinline__ double2 coupling_of(const InstDev& I, const double2* 
     const int N = NT > 0 ? NT : I.N;
     double2 c = make_double2(0.0, 0.0);
     const int jr = row - I.so, jc = col - I.so;
+    // 24-bit products (2 K + 1) N < 2^24: full-rate multiplies
     if (jr >= 0 && jr < I.K) {
-        const double2 v = cp[(jr * 2 + 0) * N + col];
+        const double2 v = cp[__mul24(jr * 2 + 0, N) + col];
         c.x += v.x;
         c.y += v.y;
     }
     if (jc >= 0 && jc < I.K) {
-        const double2 v = cp[(jc * 2 + 1) * N + row];
+        const double2 v = cp[__mul24(jc * 2 + 1, N) + row];
         c.x += v.x;
         c.y += v.y;
     }
