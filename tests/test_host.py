@@ -297,3 +297,19 @@ def test_batch_plan_system_size_range():
     with pytest.raises(E.EtoHeomError) as ei:
         E.batch_plan([item(2048, L=7)])
     assert "2^31" in str(ei.value)
+
+
+def test_float_reciprocal_splits_are_exact():
+    """The stage kernels split a coupling-gather entry e by a runtime divisor n as
+    int((float(e) + 0.5f) * (1.0f / float(n))) in IEEE float32 (device.hpp entry_split and
+    gather_couplings).  Over the ranges they use, that equals e // n: e < 2 K N <= 2^13 for
+    n = N (17..64) and n = 2 N (34..128), and the quotient by K (1..64) of values below 2^16.
+    numpy's float32 rounds as the device does (correctly rounded division, multiply, add)."""
+    f = np.float32
+    for divisors, limit in ((range(17, 65), 1 << 13), (range(34, 129, 2), 1 << 13), (range(1, 65), 1 << 16)):
+        e = np.arange(limit, dtype=np.int64)
+        ef = e.astype(np.float32)
+        for n in divisors:
+            inv = f(1.0) / f(n)
+            q = ((ef + f(0.5)) * inv).astype(np.int64)   # truncation, as the device's conversion
+            assert np.array_equal(q, e // n), n
