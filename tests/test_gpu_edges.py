@@ -5,7 +5,8 @@ normwise relative, FP64):
   dipoles; N = 60 with a banded H and sparse dipoles on every launch candidate (the
   sparse kernels at eight rows per lane);
 * N > 64 on the wide kernel, one or several workgroups per ADO: the 11-site aggregate
-  (N = 67), random systems of N = 128 and 256, the 16-site aggregate (N = 137);
+  (N = 67), random systems of N = 128 and 256, the 16-site aggregate (N = 137), and N = 67,
+  100 and 137 together in one group;
 * L = 0: the hierarchy is rho_0 alone (no neighbour links);
 * K = 0: no bath at all (d = 0, one ADO), including N = 1;
 * instances of different lengths and N classes in one batch, each read out to its own
@@ -268,6 +269,30 @@ def test_n137_sixteen_site_aggregate_every_candidate(ctx):
     p = strong(configs.aggregate(16, 1, nsteps=8), taus=(0.5, 1.5, 2.5))
     assert p.N == 137 and p.num_ado() == 33
     every_candidate(ctx, p, [1, 6])
+
+
+def test_mixed_large_n_in_one_group_every_candidate(ctx):
+    """N = 67, 100 and 137 in one batch: one N > 64 group, whose spread launch sizes its
+    workgroups per ADO for N = 137 (six), so the smaller instances' extra workgroups find no
+    tile and exit. Each instance against the oracle on every candidate."""
+    ps = [strong(configs.aggregate(11, 1, nsteps=8), taus=(0.5, 1.5, 2.5)),
+          random_system(100, 100, 8, taus=(0.5, 1.5, 2.5)),
+          strong(configs.aggregate(16, 1, nsteps=8), taus=(0.5, 1.5, 2.5))]
+    assert [p.N for p in ps] == [67, 100, 137]
+    b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)} for p in ps])
+    cands = b.candidates()
+    assert len({g for g, _, _, _ in cands}) == 1
+    assert sorted(t for _, _, k, t in cands if k == "stage_mfma_wide") == [1, 6]
+    refs = [O.twod(oproblem(p), nv=(1, 1, 1), want_rho=True) for p in ps]
+    for g, c, kern, t in cands:
+        print(f"mixed N > 64 candidate {c}: {kern} {t}", flush=True)
+        b.select(g, c)
+        b.run()
+        b.sync()
+        for i, (Pr, rr) in enumerate(refs):
+            assert rel(b.output(i), Pr) < TOL, (kern, t, ps[i].N)
+            assert rel(b.rho(i), rr) < TOL, (kern, t, ps[i].N)
+    b.close()
 
 
 def test_n256_random_hermitian_system(ctx):
