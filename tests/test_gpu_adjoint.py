@@ -176,6 +176,47 @@ def test_adjoint_generic_large_n(ctx):
         assert abs(got - ref) / max(abs(ref), 1e-12) < 1e-6, (name, idx, got, ref)
 
 
+def test_adjoint_above_n64(ctx):
+    """N = 67 (a random Hermitian system with three baths of two modes, L = 1): the N > 64
+    group runs the wide forward kernel and the generic adjoint kernel, whose LDS holds lambda
+    and rho (2 N^2 + 2 K N complex: 147 KB here)."""
+    base = configs.chain(L=1, nsteps=24)
+    rng = np.random.default_rng(67)
+    N = 67
+    A = rng.normal(size=(N, N)) * 0.02
+    H = np.diag(rng.uniform(2.2, 2.45, N)) + (A + A.T) / 2
+    H[0, :] = 0.0
+    H[:, 0] = 0.0
+    X = np.zeros((4, N, N), complex)
+    for q in range(4):
+        v = rng.normal(size=N - 1)
+        X[q, 0, 1:] = v
+        X[q, 1:, 0] = v
+    pul = base.pulses.copy()
+    pul[:, 1] = [2.0, 5.0, 8.0]
+    pul[:, 2] = 3.0
+    pul[:, 0] *= 3e5
+    p = E.Problem(H=H.astype(complex), X=X, alpha=base.alpha[:3], alpha_t=base.alpha_t[:3], gamma=base.gamma[:3],
+                  L=1, site_offset=base.site_offset, pulses=pul, t_start=base.t_start, t_end=base.t_end,
+                  step_size=base.step_size, print_step=base.print_step)
+    P0, _ = O.twod(oproblem(p), nv=(1, 1, 1))
+    rng = np.random.default_rng(5)
+    w = rng.normal(size=len(P0)) + 1j * rng.normal(size=len(P0))
+    b = E.Batch(ctx, [{"problem": p, "H": p.H, "X": p.X, "nv": (1, 1, 1)}], adjoint=8)
+    assert "stage_mfma_wide" in {k for _, _, k, _ in b.candidates()}
+    b.run()
+    b.sync()
+    assert np.max(np.abs(b.output(0) - P0)) / np.max(np.abs(P0)) < 1e-10
+    b.backward([w])
+    g = b.grad(0)
+    for name, idx, part in [("H", (1, 1), "re"), ("H", (2, 40), "re"), ("X", (0, 0, 5), "re"),
+                            ("gamma", (1, 1), "re"), ("alpha", (2, 0), "re")]:
+        ref = fd(p, w, name, idx, part)
+        got = adj_value(g, name, idx, part)
+        assert abs(got - ref) / max(abs(ref), 1e-12) < 1e-6, (name, idx, got, ref)
+    b.close()
+
+
 def test_adjoint_with_pipelined_forward_candidate(ctx):
     """The persistent pipelined N <= 32 kernel has no fused step tail: with it selected, the
     adjoint's forward runs unfused steps (a prologue per step) and the gradient agrees with
