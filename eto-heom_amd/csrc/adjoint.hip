@@ -488,7 +488,11 @@ __global__ __launch_bounds__(256) void adj_small(AdjArgs J) {
     }
 }
 
-// any N <= 64: one ADO per workgroup (VALU); dynamic LDS: l-input, Y, couplings
+// N > 16 outside the MFMA kernel's range: one ADO per workgroup (VALU).  Dynamic LDS: the
+// lambda input and Y (G false), the coupling rows/columns, the bath sums.  G: lambda and Y
+// are read where they lie (neither is written by the stage: its outputs go to other
+// buffers), for N whose 2 N^2 complex do not fit the LDS.
+template <bool G>
 __global__ __launch_bounds__(256) void adj_generic(AdjArgs J) {
     const StageArgs& A = J.S;
     extern __shared__ double2 smem[];
@@ -499,17 +503,19 @@ __global__ __launch_bounds__(256) void adj_generic(AdjArgs J) {
     if (step < 0 || step >= I.nsteps) return;
     J.Yf += (long long)step * J.traj;
     const int N = I.N, NN = N * N, d = I.d;
-    double2* sR = smem;
-    double2* sY = smem + NN;
-    double2* sC = smem + 2 * NN;
-    double* sU = reinterpret_cast<double*>(smem + 2 * NN + 2 * I.K * N);
     __shared__ double2 red[256];
     const int a0 = A.tile_ado0[tile];
     const int st = A.stage;
     const long long base = I.state_off + (long long)a0 * NN;
-    for (int e = threadIdx.x; e < NN; e += blockDim.x) {
-        sR[e] = A.rin[base + e];
-        sY[e] = J.Yf[base + e];
+    const double2* sR = G ? A.rin + base : smem;
+    const double2* sY = G ? J.Yf + base : smem + NN;
+    double2* sC = G ? smem : smem + 2 * NN;
+    double* sU = reinterpret_cast<double*>(sC + 2 * I.K * N);
+    if (!G) {
+        for (int e = threadIdx.x; e < NN; e += blockDim.x) {
+            smem[e] = A.rin[base + e];
+            smem[NN + e] = J.Yf[base + e];
+        }
     }
     for (int e = threadIdx.x; e < 6 * d; e += blockDim.x) sU[e] = 0.0;
     gather_couplings(A, I, a0, 1, sC, threadIdx.x, blockDim.x);
@@ -940,10 +946,10 @@ struct AdjState {
 
 void adj_destroy(AdjState* a) { delete a; }
 
-// dynamic LDS of adj_generic for a group: rho and lambda (N^2 each), the coupling
-// rows/columns (2 K N), six d-vectors of bath sums
-static size_t adj_generic_lds(const Group& g) {
-    return (size_t)(2 * g.maxnn + g.maxkn) * sizeof(double2) + (size_t)6 * g.maxd * sizeof(double);
+// dynamic LDS of adj_generic for a group: rho and lambda (N^2 each; none with G), the
+// coupling rows/columns (2 K N), six d-vectors of bath sums
+static size_t adj_generic_lds(const Group& g, bool G = false) {
+    return (size_t)((G ? 0 : 2 * g.maxnn) + g.maxkn) * sizeof(double2) + (size_t)6 * g.maxd * sizeof(double);
 }
 
 // dynamic LDS of adj_mfma64: l at stride N + 1, the coupling rows/columns, six d-vectors
@@ -983,8 +989,10 @@ static int launch_adj(const Group& g, const AdjArgs& base, hipStream_t s) {
                     hipLaunchKernelGGL(adj_mfma64<false>, grid, dim3(512), lds, s, a);
                 }
             }
+            else if (g.adj_global)
+                hipLaunchKernelGGL(adj_generic<true>, grid, block, adj_generic_lds(g, true), s, a);
             else
-                hipLaunchKernelGGL(adj_generic, grid, block, adj_generic_lds(g), s, a);
+                hipLaunchKernelGGL(adj_generic<false>, grid, block, adj_generic_lds(g), s, a);
             break;
         }
     }
@@ -1048,18 +1056,20 @@ int batch_enable_adjoint(Batch* b, int C) {
             stot += gacc_size(I.N * I.N, I.d);
         }
     }
-    // the N > 16 adjoint kernels stage lambda (adj_generic: also rho) and the coupling
-    // rows/columns in LDS
+    // the N > 16 adjoint kernels stage lambda (adj_generic: also Y) and the coupling
+    // rows/columns in LDS; adj_generic reads lambda and Y in place where they do not fit
     for (Group& g : im->groups) {
         if (g.nt >= 1 && g.nt <= 16) continue;
         const bool mf = adj_uses_mfma64(g);
-        const size_t sh = mf ? adj_mfma64_lds(g) : adj_generic_lds(g);
+        g.adj_global = !mf && (long long)adj_generic_lds(g) > b->ctx->lds_per_block;
+        const size_t sh = mf ? adj_mfma64_lds(g) : adj_generic_lds(g, g.adj_global);
         if ((long long)sh > b->ctx->lds_per_block)
             return fail(EH_ESHAPE, "adjoint: an N > 16 instance needs " + std::to_string(sh) + " bytes of LDS per " +
-                                       "workgroup (" + (mf ? "N (N+1)" : "2 N^2") + " + 2 K N complex + 48 d); " +
+                                       "workgroup (" + (mf ? "N (N+1) + " : "") + "2 K N complex + 48 d); " +
                                        "the device has " + std::to_string(b->ctx->lds_per_block));
         if (sh > 65536)
-            HIPCHK(hipFuncSetAttribute(mf ? (const void*)adj_mfma64<false> : (const void*)adj_generic,
+            HIPCHK(hipFuncSetAttribute(mf ? (const void*)adj_mfma64<false>
+                                          : (g.adj_global ? (const void*)adj_generic<true> : (const void*)adj_generic<false>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
         // two ADOs per workgroup where both halves' LDS fits (ETOHEOM_ADJ_NO_PAIR=1: one)
         const size_t sh2 = 2 * ((sh + 15) & ~(size_t)15);

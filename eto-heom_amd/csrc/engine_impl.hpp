@@ -81,6 +81,7 @@ struct Group {
     std::vector<double> cand_ms;  // autotuner timings per candidate (ms per RK4 step), empty if not tuned
     int sel = 0;                  // the candidate the forward graph launches
     bool adj_pair = false;        // adj_mfma64 runs two ADOs per workgroup (set by the adjoint's enable)
+    bool adj_global = false;      // adj_generic reads lambda and Y in place (they do not fit the LDS)
 };
 
 // Host side of a batch (build_host_batch): the tables batch_create uploads, the
