@@ -786,7 +786,10 @@ __device__ __forceinline__ void sx_rows(const StageArgs& A, const InstDev& I, in
     // the wave index (and so every row a = w + 8 i) as a scalar: row offsets go to the SALU
     // (stage 1 measured slower with it)
     const int tid = threadIdx.x, b = tid & 63, w = S == 1 ? tid >> 6 : __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int bc = b < N ? b : 0, l16 = b & (CAP - 1);
+    // idle lanes (b >= N) read what the first lane of their 16-lane group reads: a 128-bit LDS
+    // read is served 16 lanes at a time, and element 0 of a row or column would share banks
+    // with the group's own elements (a 2-way conflict in every entry's reads at N = 56)
+    const int bc = b < N ? b : ((b & ~15) < N ? (b & ~15) : 0), l16 = b & (CAP - 1);
     const double2* V = A.V + I.v_off;
     // the state operand of the lane's elements (Y in stage 2, ACC in stage 3; stage 1 loads Y
     // after the rows)
