@@ -216,6 +216,39 @@ def test_cli_2des_prints_reference_format(ctx, tmp_path):
     assert out.stdout == want and len(P) == 6
 
 
+def test_cli_2des_eleven_site_key(ctx, tmp_path):
+    """The key-file driver above N = 64: an 11-site .key (SIZE 12: ground + 11 singles, and
+    the 55 doubles that construct_Hal adds, N = 67), L = 1, 17 steps with three strong pulses
+    inside them (E0 = 100 cm^-1: the third-order signal is then 3e-3 of the first-order terms
+    it is the difference of, so their 1e-10 agreement stays far below the print's digits).
+    etoheom_2des prints p111 - p110 - p101 in the reference's format, and its values match the
+    oracle fed the same realised system to the print's 7 digits."""
+    import subprocess
+    n = 11
+    eps = 12000.0 + 100.0 * np.arange(n)
+    J = np.zeros((n, n))
+    for i in range(n - 1):
+        J[i, i + 1] = J[i + 1, i] = 100.0
+    rng = np.random.default_rng(11)
+    mu = rng.normal(size=(n, 3))
+    mu /= np.linalg.norm(mu, axis=1, keepdims=True)
+    dip = [(0, i + 1, *mu[i]) for i in range(n)]
+    F = configs.OHMIC_35_100_77
+    txt = configs.key_text(n + 1, n, 1, configs._sites_H(eps, J), dip, configs.ohmic_modes(), F["lambda_line"],
+                           [(100.0, t, 3.0, 12050.0) for t in (0.5, 1.5, 2.5)], (0.0, 8.0, 0.5, 0.5))
+    key = tmp_path / "eleven.key"
+    key.write_text(txt)
+    p, _ = configs.realise(txt, 0)
+    assert p.N == 67
+    out = subprocess.run(["eto-heom_amd/bin/etoheom_2des", str(key)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    got = np.array([complex(float(a), float(b)) for a, b in (ln.split() for ln in out.stdout.splitlines())])
+    q = oproblem(p)
+    S = O.twod(q, nv=(1, 1, 1))[0] - O.twod(q, nv=(1, 1, 0))[0] - O.twod(q, nv=(1, 0, 1))[0]
+    assert len(got) == len(S) > 0 and np.max(np.abs(S)) > 0
+    assert np.max(np.abs(got - S)) <= 2e-6 * np.max(np.abs(S))
+
+
 def test_cli_dynamics_prints_reference_format(ctx):
     import subprocess
     out = subprocess.run(["eto-heom_amd/bin/etoheom_dynamics", "tests/golden/key_0_0.key"], capture_output=True,
