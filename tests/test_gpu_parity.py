@@ -62,6 +62,34 @@ def test_dynamics_key_matches_oracle(ctx):
     assert rel(pop, popr) < TOL
 
 
+def test_dynamics_seventy_site_key_matches_oracle(ctx, tmp_path):
+    """Population dynamics above N = 64: a 70-site chain key (SIZE 70, sites only, as the
+    reference's dynamics keys are: N = 70, site 1 excited at t = 0) with baths on five sites
+    (K = 5, L = 1, 11 ADOs), V = H and site offset 0, three site populations over 40 steps
+    against the oracle."""
+    n = 70
+    eps = 12000.0 + 10.0 * np.arange(n)
+    J = np.zeros((n, n))
+    for i in range(n - 1):
+        J[i, i + 1] = J[i + 1, i] = 100.0
+    dip = [(0, 1, 1.0, 0.0, 0.0)]
+    F = configs.OHMIC_35_100_77
+    txt = configs.key_text(n, 5, 1, np.diag(eps) + J, dip, configs.ohmic_modes(), F["lambda_line"],
+                           [(1e-4, t, 15.0, 12050.0) for t in (40.0, 90.0, 190.0)], (0.0, 20.0, 0.5, 0.5))
+    key = tmp_path / "seventy_dyn.key"
+    key.write_text(txt)
+    k = E.Param.from_key(str(key), E.MODE_DYNAMICS)
+    H = k.get("Hal")
+    assert H.shape == (70, 70)
+    pop = E.dynamics_solver(ctx, k, H, sites=(1, 2, 3))
+    q = O.Problem(H=H, alpha=k.get("alpha"), alpha_t=k.get("alpha_t"), gamma=k.get("gamma"), L=k.L, site_offset=0,
+                  t_start=0, t_end=20.0, step_size=0.5, print_step=0.5)
+    popr, _ = O.dynamics(q, sites=(1, 2, 3))
+    assert pop.shape == popr.shape == (3, 41)
+    assert np.min(np.max(np.abs(popr[1:]), axis=1)) > 1e-3
+    assert rel(pop, popr) < TOL
+
+
 @pytest.mark.parametrize("n_sites,L", [(3, 4), (4, 3)])
 def test_trimer_tetramer_match_oracle(ctx, n_sites, L):
     p = configs.aggregate(n_sites, L, sigma=100.0, nsteps=300, taus=(20.0, 40.0, 60.0))
